@@ -1,0 +1,238 @@
+/*
+ * dmf.h — C-ABI of the MI355X dynamic max-flow engine.
+ *
+ * Plain C types only (int32/int64 arrays, sizes, opaque handles). No C++ or
+ * torch types cross this boundary and no exception escapes it: every entry
+ * point returns an int status (DMF_OK = 0, negative on error).
+ *
+ * The reference (dyn_maxflow) has no library API; its interface is the pair
+ * of on-disk formats, the bench drivers' argv/stdout, and two in-process host
+ * functions.  Each declaration below names the reference code it replaces.
+ */
+#ifndef DMF_H
+#define DMF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define DMF_OK 0
+#define DMF_EINVAL (-1)   /* bad argument (vertex id out of range, s == t, ...) */
+#define DMF_EIO (-2)      /* file cannot be opened / short read / short write   */
+#define DMF_ENOARC (-3)   /* an update names an arc absent from the CSR         */
+#define DMF_ENOMEM (-4)   /* host or device allocation failed                   */
+#define DMF_EDEVICE (-5)  /* HIP runtime error                                  */
+#define DMF_ENOCONV (-6)  /* round cap reached before convergence               */
+#define DMF_EFORMAT (-7)  /* file contents inconsistent (negative sizes, ...)   */
+#define DMF_EADMIT (-8)   /* serving: request exceeds the admission budget      */
+
+/* Human-readable text of a status code (static storage). */
+const char* dmf_strerror(int status);
+
+/* ---------------------------------------------------------------- host CSR */
+/*
+ * Bidirectional CSR exactly as the reference stores it
+ * (code/graph.hpp:21-30, written by input_create/create_bin.cpp:18-64):
+ *   row[n+1]  offsets            (graph::indexofNodes)
+ *   col[m]    arc head           (graph::edgeList)
+ *   cap[m]    arc capacity       (graph::edgeLen)
+ *   rev[m]    index of the reverse arc (graph::parallel_edge)
+ * Memory returned by the dmf_* producers is owned by the library and released
+ * with dmf_csr_free.
+ */
+typedef struct dmf_csr {
+  int32_t n;
+  int32_t m;
+  int32_t* row;
+  int32_t* col;
+  int32_t* cap;
+  int32_t* rev;
+} dmf_csr;
+
+void dmf_csr_free(dmf_csr* g);
+/* Release any array returned by a dmf_* producer (update arrays, edge lists). */
+void dmf_free(void* p);
+
+/* Replaces graph::loadGraph (code/graph.hpp:202-223).  Layout, all int32 LE:
+ * [n][m][row n+1][col m][cap m][rev m].  Unlike the reference, short reads and
+ * inconsistent headers are reported (DMF_EIO / DMF_EFORMAT). */
+int dmf_read_graph_bin(const char* path, dmf_csr* out);
+
+/* Replaces writeCSR (input_create/create_bin.cpp:18-64); byte-identical output. */
+int dmf_write_graph_bin(const char* path, const dmf_csr* g);
+
+/* Replaces readEdgeList + writeCSR's construction (input_create/create_bin.cpp:67-124):
+ * arcs in first-appearance order per tail, duplicate (u,v) capacities summed,
+ * a reverse arc of capacity 0 added when absent, self-loops dropped,
+ * n = max id + 1.  Hash-indexed, so linear in k (the reference's per-edge linear
+ * search is quadratic on hubs).  k = number of input edges. */
+int dmf_build_csr(int64_t k, const int32_t* u, const int32_t* v, const int32_t* w,
+                  dmf_csr* out);
+
+/* Replaces loadUpdateFile (code/update.hpp:52-89).  Layout, all int32 LE:
+ * [k][src k][dst k][w k]; w is the new absolute capacity of arc src->dst.
+ * Returns the k records as stored (self-loops included; the engine skips them,
+ * as the reference reader does at code/update.hpp:82). */
+int dmf_read_updates_bin(const char* path, int32_t* k, int32_t** src, int32_t** dst,
+                         int32_t** w);
+
+/* Replaces input_create/create_update_bin.cpp:11-52 (the .txt -> .bin step);
+ * byte-identical output for the same records. */
+int dmf_write_updates_bin(const char* path, int32_t k, const int32_t* src,
+                          const int32_t* dst, const int32_t* w);
+
+/* ------------------------------------------------------ seeded input tools */
+/* (No reference counterpart with a seed: input_create/update_gen.cpp:31-32
+ * seeds from std::random_device.  These are the reproducible replacements.) */
+
+/* Directed R-MAT edge list: 2^scale * edge_factor samples, quadrant
+ * probabilities a,b,c (d = 1-a-b-c), no noise; self-loops and duplicate (u,v)
+ * dropped (first kept); capacities uniform in [cap_lo, cap_hi].  xoshiro256**
+ * seeded through splitmix64. */
+int dmf_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                 uint64_t seed, int32_t cap_lo, int32_t cap_hi, int64_t* k,
+                 int32_t** u, int32_t** v, int32_t** w);
+
+/* rows x cols 4-neighbour grid, id = r*cols + c, every neighbour pair in both
+ * directions with independent capacities uniform in [cap_lo, cap_hi]. */
+int dmf_gen_grid(int32_t rows, int32_t cols, uint64_t seed, int32_t cap_lo,
+                 int32_t cap_hi, int64_t* k, int32_t** u, int32_t** v, int32_t** w);
+
+/* s = argmax out-degree, t = argmax in-degree with t != s, ties -> lowest id,
+ * degrees counted over the input edge list. */
+int dmf_pick_source_sink(int32_t n, int64_t k, const int32_t* u, const int32_t* v,
+                         int32_t* s, int32_t* t);
+
+/* One update batch of `count` distinct input edges, sampled without replacement.
+ * Each is an increase with probability inc_frac, else a decrease; the new
+ * capacity is uniform in [cur+1, cap_hi] (increase) or [cap_lo, cur-1]
+ * (decrease); an edge already at that bound is skipped and another drawn.
+ * cur_w (length k) holds the current capacities and, if apply != 0, is updated
+ * in place so successive batches build on each other.  Outputs have `*out_k`
+ * records (< count only when the graph runs out of eligible edges). */
+int dmf_gen_updates(int64_t k, const int32_t* u, const int32_t* v, int32_t* cur_w,
+                    int64_t count, double inc_frac, int32_t cap_lo, int32_t cap_hi,
+                    uint64_t seed, int apply, int32_t* out_k, int32_t** src,
+                    int32_t** dst, int32_t** w);
+
+/* ------------------------------------------------------------------ engine */
+/* Variants: the three reference paths (code/README.md:36-42). */
+#define DMF_VARIANT_DATA 0  /* worklist-driven       (code/dyn_data.h)  */
+#define DMF_VARIANT_TOPO 1  /* all-vertex rounds     (code/dyn_topo.h)  */
+#define DMF_VARIANT_PP 2    /* push-pull over a cut  (code/dyn_pp.h)    */
+
+#define DMF_SCHED_FIXED 0   /* global relabel every `relabel_interval` rounds */
+#define DMF_SCHED_LEARNED 1 /* on-device MLP decides relabel + next variant   */
+
+typedef struct dmf_params {
+  int32_t variant;          /* DMF_VARIANT_*                                     */
+  int32_t kc;               /* push/relabel cycles per vertex per round (>= 1)   */
+  int32_t relabel_interval; /* rounds between global relabels (1 = reference)   */
+  int32_t scheduler;        /* DMF_SCHED_*                                       */
+  int32_t max_rounds;       /* safety cap on rounds per solve (0 = default)     */
+  int32_t mode;             /* 0 = persistent one-launch solve, 1 = stepped     */
+} dmf_params;
+
+/* Fills the reference's fixed schedule: data variant, kc = max(1, m/n),
+ * global relabel every round (code/dyn_data_bench.cu:529-562, :65-79). */
+void dmf_params_default(const dmf_csr* g, dmf_params* p);
+
+/* Work counters of the last solve (static or batch), accumulated on device. */
+typedef struct dmf_counters {
+  uint64_t arc_visits;     /* adjacency entries examined                   */
+  uint64_t vertex_visits;  /* vertices examined by any phase               */
+  uint64_t pushes;         /* push operations                              */
+  uint64_t relabels;       /* local relabels                               */
+  uint64_t repairs;        /* label-repair saturations                     */
+  uint64_t bfs_levels;     /* levels over all global relabels              */
+  uint64_t global_relabels;
+  uint64_t rounds;         /* push/repair rounds                           */
+  uint64_t updates;        /* update records applied                       */
+  uint64_t atomics;        /* global atomic RMW operations issued          */
+  uint64_t launches;       /* kernel launches                              */
+  double device_ms;        /* HIP-event span of the solve on its stream    */
+} dmf_counters;
+
+typedef struct dmf_engine dmf_engine;
+
+/* Copies the CSR into a device-resident residual graph (cf. the per-call
+ * allocation + H2D copies of code/dyn_data_bench.cu:182-217, kept resident
+ * here across batches).  s,t are fixed for the engine's life. */
+int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out);
+int dmf_engine_destroy(dmf_engine* e);
+
+/* Run all later work on this HIP stream (hipStream_t passed as void*); NULL =
+ * the engine's own stream. */
+int dmf_engine_set_stream(dmf_engine* e, void* hip_stream);
+
+/* Restore original capacities and zero flow (no solve). */
+int dmf_engine_reset(dmf_engine* e);
+
+/* Static solve from zero flow: replaces staticMaxFlow
+ * (code/dyn_data_bench.cu:8-97, code/dyn_topo_bench.cu:16-75,
+ * code/dyn_pp_bench.cu:155-239).  *flow = max-flow value (deficits folded). */
+int dmf_engine_static_solve(dmf_engine* e, const dmf_params* p, int64_t* flow);
+
+/* Apply one batch of absolute capacity updates and re-converge: replaces
+ * OnAdd_kernel + Incremental (code/dyn_data.h:186-206,
+ * code/dyn_data_bench.cu:98-170; topo/pp counterparts).  Host arrays; the copy
+ * to HBM is outside the device-time span.  Duplicate arcs in one batch: the
+ * last record wins.  A missing arc fails the whole batch with DMF_ENOARC and
+ * leaves the state untouched; self-loops are skipped. */
+int dmf_engine_apply_batch(dmf_engine* e, int32_t k, const int32_t* src,
+                           const int32_t* dst, const int32_t* w,
+                           const dmf_params* p, int64_t* flow);
+
+/* Same, with the three arrays already resident in device memory. */
+int dmf_engine_apply_batch_device(dmf_engine* e, int32_t k, const int32_t* d_src,
+                                  const int32_t* d_dst, const int32_t* d_w,
+                                  const dmf_params* p, int64_t* flow);
+
+/* Current max-flow value (excess[t] plus the deficits of v != s, as
+ * recalculate_max_flow_kernel_39, code/dyn_data.h:208-215). */
+int dmf_engine_flow(dmf_engine* e, int64_t* flow);
+
+/* Source side of the minimum cut S_min (vertices reachable from s in the
+ * residual graph of a maximum flow), side[v] = 1 for v in S_min.  No reference
+ * counterpart (the reference never reports a cut set). */
+int dmf_engine_min_cut(dmf_engine* e, uint8_t* side);
+
+/* Copy device state to host for checking: res/cap per arc (m), excess per
+ * vertex (n), height per vertex (n).  Any pointer may be NULL. */
+int dmf_engine_read_state(dmf_engine* e, int32_t* res, int32_t* cap, int64_t* excess,
+                          int32_t* height);
+
+int dmf_engine_counters(dmf_engine* e, dmf_counters* c);
+
+/* Bytes of device memory held by this engine. */
+int64_t dmf_engine_device_bytes(const dmf_engine* e);
+
+/* ------------------------------------------ reference in-process entry points */
+/* int recalculate_max_flow(graph&, std::vector<update>, int batchSize, int s,
+ * int t, int kernel_cycles)  (code/dyn_data_bench.cu:171-298,
+ * code/dyn_topo_bench.cu:120-239, code/dyn_pp_bench.cu:469-621):
+ * fresh device state, static solve, the whole update list as one batch,
+ * deficit fold, flow.  ms_static / ms_batch receive the two HIP-event spans
+ * the reference prints ("static GPU Time", "DYN TIME"); either may be NULL. */
+int dmf_recalculate_max_flow(const dmf_csr* g, int32_t k, const int32_t* src,
+                             const int32_t* dst, const int32_t* w, int32_t s,
+                             int32_t t, int32_t kc, int32_t variant, int64_t* flow,
+                             double* ms_static, double* ms_batch);
+
+/* int static_recalculate(...)  (code/dyn_data_bench.cu:325-492 and
+ * counterparts): capacities set from the updates, then one static solve from
+ * scratch.  ms receives "STATIC UPDATE GPU Time". */
+int dmf_static_recalculate(const dmf_csr* g, int32_t k, const int32_t* src,
+                           const int32_t* dst, const int32_t* w, int32_t s, int32_t t,
+                           int32_t kc, int32_t variant, int64_t* flow, double* ms);
+
+/* Number of HIP devices visible (0 on a machine without a GPU). */
+int dmf_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DMF_H */

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP engine (through the C-ABI) against the oracle.
+
+Bar: bit-exact max-flow value and S_min (vertices reachable from s in the
+residual graph of a maximum flow) after the static solve and after every update
+batch, plus the residual-state invariants (0 <= res, pair consistency,
+excess = net inflow).  Runs every variant (data, topo, pp) under both drivers
+(persistent one-launch solve and stepped per-phase launches)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from util import caps_after, rmat_case
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = ["data", "topo", "pp"]
+MODES = [0, 1]  # 0 persistent, 1 stepped
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check_engine(eng, g, cap, s, t, flow, where):
+    fd, side = oracle.dinic(g, s, t, cap=cap)
+    assert flow == fd, f"{where}: flow {flow} != dinic {fd}"
+    smin = eng.min_cut()
+    assert np.array_equal(smin, side), f"{where}: S_min differs on {(smin != side).sum()} vertices"
+    res, capd, ex, _ = eng.state()
+    assert np.array_equal(capd, cap), f"{where}: capacities differ"
+    bad = oracle.check_state(g, res, capd, ex)
+    assert not bad, f"{where}: {bad}"
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_golden_johnson(gpu, golden_dir, expected, variant, mode):
+    dmf = gpu
+    g = dmf.read_graph_bin(os.path.join(golden_dir, "johnson.bin"))
+    J = expected["johnson"]
+    s, t = J["s"], J["t"]
+    p = dmf.default_params(g, variant=variant, mode=mode)
+    say(f"[johnson] variant={variant} mode={mode} static")
+    with dmf.Engine(g, s, t) as eng:
+        assert eng.static_solve(p) == J["static_flow"]
+        assert np.flatnonzero(eng.min_cut()).tolist() == J["static_smin"]
+    for b in J["batches"]:  # independent files on the original graph (code/README.md:24)
+        say(f"[johnson] variant={variant} mode={mode} batch {b['file']}")
+        src, dst, w = dmf.read_updates_bin(os.path.join(golden_dir, b["file"]))
+        with dmf.Engine(g, s, t) as eng:
+            eng.static_solve(p)
+            assert eng.apply_batch(src, dst, w, p) == b["flow"]
+            assert np.flatnonzero(eng.min_cut()).tolist() == b["smin"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rmat_successive_batches(gpu, variant, mode):
+    dmf = gpu
+    for scale, seed in ((9, 1), (12, 2)):
+        g, s, t, (u, v, w) = rmat_case(scale, 16, seed)
+        p = dmf.default_params(g, variant=variant, mode=mode)
+        say(f"[rmat] scale={scale} seed={seed} variant={variant} mode={mode} static")
+        with dmf.Engine(g, s, t) as eng:
+            f = eng.static_solve(p)
+            check_engine(eng, g, g.cap, s, t, f, "static")
+            cur = w.copy()
+            applied = []
+            for b in range(1, 6):
+                src, dst, nw = dmf.gen_updates(u, v, cur, len(u) * b // 100 + 1, seed * 1000 + b)
+                applied.append((src, dst, nw))
+                say(f"[rmat] scale={scale} variant={variant} mode={mode} batch {b} k={len(src)}")
+                f = eng.apply_batch(src, dst, nw, p)
+                check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rmat14_config_ten_one_percent_batches(gpu, variant):
+    """Small plain configuration: RMAT-14, EF16, seed 1, 10 batches of 1%."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(14, 16, 1)
+    p = dmf.default_params(g, variant=variant)
+    say(f"[rmat14] variant={variant}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b in range(10):
+            src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 100, 100 + b)
+            applied.append((src, dst, nw))
+            f = eng.apply_batch(src, dst, nw, p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_reference_entry_points_agree(gpu, golden_dir, expected, variant):
+    """dynamic == static recompute (the reference's PANIC check,
+    code/dyn_data_bench.cu:535-539) and both equal Dinic."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(11, 16, 4)
+    kc = max(1, g.m // g.n)
+    cur = w.copy()
+    src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 20, 77, apply=False)
+    say(f"[entry points] variant={variant}")
+    dyn, ms_s, ms_b = dmf.recalculate_max_flow(g, src, dst, nw, s, t, kc, variant)
+    stat, ms = dmf.static_recalculate(g, src, dst, nw, s, t, kc, variant)
+    fd, _ = oracle.dinic(g, s, t, cap=caps_after(g, [(src, dst, nw)]))
+    assert dyn == stat == fd
+    assert ms_s > 0 and ms_b > 0 and ms > 0
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batch_edge_cases(gpu, mode):
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(10, 8, 3)
+    p = dmf.default_params(g, mode=mode)
+    say(f"[edge cases] mode={mode}")
+    with dmf.Engine(g, s, t) as eng:
+        f0 = eng.static_solve(p)
+        # empty batch
+        assert eng.apply_batch([], [], [], p) == f0
+        # self-loop records are skipped
+        assert eng.apply_batch([s, 3], [s, 3], [7, 9], p) == f0
+        # an absent arc fails the whole batch and changes nothing
+        tails = g.tails()
+        present = set(zip(tails.tolist(), g.col.tolist()))
+        a = next(x for x in range(g.n) if (s, x) not in present and x != s)
+        rc = eng.apply_batch_status([int(u[0]), s], [int(v[0]), a], [1, 5], p)
+        assert rc == dmf.ENOARC
+        res, cap, ex, _ = eng.state()
+        assert np.array_equal(cap, g.cap)
+        # out-of-range vertex id
+        rc = eng.apply_batch_status([g.n + 5], [0], [1], p)
+        assert rc == dmf.EINVAL
+        assert eng.flow() == f0
+        # duplicate records for one arc: the last wins
+        a, b = int(u[1]), int(v[1])
+        f = eng.apply_batch([a, a, a], [b, b, b], [100, 1, 42], p)
+        cap_exp = caps_after(g, [([a], [b], [42])])
+        check_engine(eng, g, cap_exp, s, t, f, "duplicates")
+        # the same arc again with an increase on every arc out of s
+        so = g.col[g.row[s]:g.row[s + 1]]
+        f = eng.apply_batch(np.full(len(so), s), so, np.full(len(so), 100), p)
+        cap_exp = caps_after(g, [([a], [b], [42]), (np.full(len(so), s), so, np.full(len(so), 100))])
+        check_engine(eng, g, cap_exp, s, t, f, "source increases")
+        # every arc into t decreased to 1
+        ti = np.flatnonzero(g.col == t)
+        f = eng.apply_batch(tails[ti], np.full(len(ti), t), np.ones(len(ti), np.int32), p)
+        cap_exp2 = cap_exp.copy()
+        cap_exp2[ti] = 1
+        check_engine(eng, g, cap_exp2, s, t, f, "sink decreases")
+
+
+def test_disconnected_source_and_sink(gpu):
+    dmf = gpu
+    g = dmf.build_csr([0, 1, 2, 3, 4], [1, 0, 3, 2, 5], [5, 5, 5, 5, 1])
+    for variant in VARIANTS:
+        for mode in MODES:
+            with dmf.Engine(g, 0, 3) as eng:
+                p = dmf.default_params(g, variant=variant, mode=mode)
+                assert eng.static_solve(p) == 0
+                assert eng.min_cut().tolist() == [1, 1, 0, 0, 0, 0]
+
+
+@pytest.mark.parametrize("interval", [2, 5])
+def test_relabel_interval_keeps_results(gpu, interval):
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 6)
+    p = dmf.default_params(g, relabel_interval=interval)
+    say(f"[interval] {interval}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b in range(3):
+            src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 30, 500 + b)
+            applied.append((src, dst, nw))
+            f = eng.apply_batch(src, dst, nw, p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+def test_counters_are_populated(gpu):
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 8)
+    p = dmf.default_params(g)
+    with dmf.Engine(g, s, t) as eng:
+        eng.static_solve(p)
+        c = eng.counters()
+        assert c["arc_visits"] > g.m // 2
+        assert c["global_relabels"] >= 1 and c["bfs_levels"] >= 1
+        assert c["device_ms"] > 0
+        cur = w.copy()
+        src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 50, 3)
+        eng.apply_batch(src, dst, nw, p)
+        c = eng.counters()
+        assert c["updates"] == len(src)
+        assert eng.device_bytes() > 20 * g.m

@@ -56,13 +56,50 @@ def make_workload(scale, ef, seed, npct):
     return g, s, t, (u, v, w), batches
 
 
+class HipBuffers:
+    """Device buffers through the HIP runtime libdmf itself is linked against
+    (a second HIP runtime in the process, such as the one bundled with a
+    PyTorch wheel, cannot share the device with it)."""
+
+    def __init__(self):
+        import ctypes
+        self.C = ctypes
+        for name in ("libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so.7", "libamdhip64.so"):
+            try:
+                self.hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def upload(self, a: np.ndarray) -> int:
+        a = np.ascontiguousarray(a, dtype=np.int32)
+        p = self.C.c_void_p()
+        if self.hip.hipMalloc(self.C.byref(p), max(4, a.nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if a.nbytes and self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) != 0:  # host -> device
+            raise RuntimeError("hipMemcpy failed")
+        self.ptrs.append(p)
+        return p.value
+
+    def sync(self):
+        self.hip.hipDeviceSynchronize()
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
+
+
 def run_gpu(g, s, t, batches, params, steps, warmup, label):
     """Returns per-batch records for `steps` timed batches after `warmup` untimed."""
-    import torch
-
     import dyn_maxflow_amd as dmf
-    dev = [tuple(torch.from_numpy(np.ascontiguousarray(a)).to("cuda") for a in b) for b in batches]
-    torch.cuda.synchronize()
+    bufs = HipBuffers()
+    dev = [tuple(bufs.upload(a) for a in b) for b in batches]
+    bufs.sync()
     recs, flows = [], {}
     done = 0
     total = steps + warmup
@@ -73,10 +110,9 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label):
             for bi, (ds, dd, dw) in enumerate(dev):
                 if done >= total:
                     break
-                torch.cuda.synchronize()
+                bufs.sync()
                 w0 = time.perf_counter()
-                f = eng.apply_batch_device(len(batches[bi][0]), ds.data_ptr(), dd.data_ptr(),
-                                           dw.data_ptr(), params)
+                f = eng.apply_batch_device(len(batches[bi][0]), ds, dd, dw, params)
                 wall = (time.perf_counter() - w0) * 1e3
                 c = eng.counters()
                 if bi in flows and flows[bi] != f:
@@ -89,6 +125,7 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label):
                     recs.append(c)
                 done += 1
             log(f"[bench] {label}: {done}/{total} batches")
+    bufs.free()
     return recs, flows
 
 

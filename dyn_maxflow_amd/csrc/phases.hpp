@@ -1,0 +1,1547 @@
+// Device side of the MI355X dynamic max-flow engine: state layout, wave64
+// helpers, the phase functions of the three push-relabel variants, the
+// stepped (one kernel per phase) kernels and the persistent cooperative
+// kernel.  Included by engine.hip only.
+//
+// Work distribution (every phase that walks adjacency lists):
+//   * degree <= kLaneMax          one lane scans the vertex serially;
+//   * kLaneMax < degree <= kWideT one wave64 strides over the adjacency;
+//   * degree > kWideT             the vertex goes to a "wide queue" and its
+//                                 adjacency is cut into kChunk-arc chunks that
+//                                 any wave of the grid picks up, so a hub of
+//                                 60k arcs is spread over ~240 waves instead
+//                                 of serialising one wave.
+// A wide queue is a pair of arrays (vertex, first chunk) filled through one
+// 64-bit atomic that reserves the slot (high half) and the chunk range (low
+// half) together, so first-chunk values are monotone in slot order and a chunk
+// finds its vertex by binary search.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+
+#include <climits>
+#include <cstdint>
+
+#include "../../include/dmf.h"
+
+namespace cg = cooperative_groups;
+
+namespace dmfdev {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+constexpr int kLaneMax = 32;
+constexpr int kWideT = 1024;
+constexpr int kChunk = 256;
+
+enum CtrIdx {
+  C_ARC = 0,
+  C_VERT,
+  C_PUSH,
+  C_RELABEL,
+  C_REPAIR,
+  C_LEVELS,
+  C_GRELABEL,
+  C_ROUNDS,
+  C_UPD,
+  C_ATOMIC,
+  C_N = 16
+};
+
+// Wide queues: 0..2 BFS frontier ring (level % 3), 3 active list, 4 pull list.
+constexpr int kQActive = 3;
+constexpr int kQPull = 4;
+constexpr int kNumQ = 5;
+
+// Device control block: loop counters, flags and work counters.
+struct Ctrl {
+  int32_t qcnt[3];    // frontier sizes, ring indexed by level % 3
+  int32_t wl_cnt;     // active list size
+  int32_t wl2_cnt;    // pull list size (push-pull)
+  int32_t part_cnt;   // stranded part size (push-pull)
+  int32_t flag[2];    // topology variant: some vertex was active (slot = round & 1)
+  int32_t status;     // DMF_OK or an error code set on device
+  int32_t missing;    // updates naming absent arcs
+  int32_t invalid;    // updates with out-of-range ids
+  int32_t visited;    // vertices labelled by the last global relabel
+  int32_t rounds;     // rounds of the last solve
+  int32_t pad[3];
+  unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
+  unsigned long long ctr[C_N];
+  long long flow;
+};
+
+struct Dev {
+  int32_t n, m, s, t;
+  const int32_t* row;
+  const int32_t* col;
+  const int32_t* rev;
+  int32_t* cap;
+  int32_t* res;
+  int32_t* rres;
+  long long* excess;
+  int32_t* height;
+  int32_t* order;
+  int32_t* wl;
+  int32_t* wl2;
+  int32_t* part;
+  uint8_t* side;
+  uint8_t* pfinal;
+  Ctrl* ctrl;
+  unsigned long long* hkey;
+  int32_t* hval;
+  unsigned long long hmask;
+  int32_t* owner;
+  // wide queues (kNumQ x wq_cap) and the static list of every wide vertex
+  int32_t* wq_vert;
+  int32_t* wq_base;
+  int32_t wq_cap;
+  const int32_t* wide_vert;
+  const int32_t* wide_base;
+  unsigned long long wide_packed;
+  // current update batch
+  const int32_t* usrc;
+  const int32_t* udst;
+  const int32_t* uw;
+  int32_t* uarc;
+  int32_t k;
+};
+
+// Solve configuration passed to every driver.
+struct Cfg {
+  int32_t variant;
+  int32_t kc;
+  int32_t interval;
+  int32_t max_rounds;
+  int32_t entry;  // 0 = static (init + saturate), 1 = batch (apply + repair)
+};
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+// ------------------------------------------------------------ small helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ int ldr(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long ldr64(const long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ldru64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void str(int32_t* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long add64(long long* p, long long v) {
+  return static_cast<long long>(
+      atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v)));
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long y = __shfl_xor(x, o);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+__device__ __forceinline__ int wave_min_i32(int x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o));
+  return x;
+}
+__device__ __forceinline__ long long wave_sum_i64(long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+__device__ __forceinline__ long long wave_incl_scan_i64(long long x) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const long long y = __shfl_up(x, o);
+    if (lane_id() >= o) x += y;
+  }
+  return x;
+}
+
+// Reserve slots for the lanes with pred set: one atomic per wave.  Must be
+// reached by every active lane of the wave.
+__device__ __forceinline__ int wave_reserve(int32_t* counter, bool pred, int* rank) {
+  const unsigned long long act = __ballot(1);
+  const unsigned long long b = __ballot(pred);
+  const int leader = __ffsll(static_cast<long long>(act)) - 1;
+  const int total = __popcll(b);
+  int base = 0;
+  if (lane_id() == leader && total) base = atomicAdd(counter, total);
+  base = __shfl(base, leader);
+  *rank = __popcll(b & ((1ull << lane_id()) - 1ull));
+  return base;
+}
+
+// Per-thread work counters, flushed once per phase with one atomic per wave.
+struct Tally {
+  unsigned long long arc = 0, vert = 0, push = 0, relabel = 0, repair = 0, atom = 0;
+  __device__ void flush(Ctrl* c) {
+    unsigned long long v[6] = {arc, vert, push, relabel, repair, atom};
+    const int idx[6] = {C_ARC, C_VERT, C_PUSH, C_RELABEL, C_REPAIR, C_ATOMIC};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      unsigned long long x = v[i];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+      if (lane_id() == 0 && x) atomicAdd(&c->ctr[idx[i]], x);
+    }
+    arc = vert = push = relabel = repair = atom = 0;
+  }
+};
+
+struct Range {
+  int gid;  // global thread index
+  int gsz;  // total threads
+  int wid;  // global wave index
+  int nw;   // total waves
+};
+
+__device__ __forceinline__ Range make_range() {
+  Range r;
+  r.gid = blockIdx.x * blockDim.x + threadIdx.x;
+  r.gsz = gridDim.x * blockDim.x;
+  r.wid = r.gid / kWave;
+  r.nw = r.gsz / kWave;
+  return r;
+}
+
+// ------------------------------------------------------------- wide queues
+struct WList {
+  const int32_t* vert;
+  const int32_t* base;
+  int slots;
+  int chunks;
+};
+
+__device__ __forceinline__ WList wlist(const Dev& D, int q, unsigned long long packed) {
+  WList L;
+  L.vert = D.wq_vert + static_cast<size_t>(q) * D.wq_cap;
+  L.base = D.wq_base + static_cast<size_t>(q) * D.wq_cap;
+  L.slots = static_cast<int>(packed >> 32);
+  L.chunks = static_cast<int>(packed & 0xffffffffu);
+  return L;
+}
+__device__ __forceinline__ WList wlist_static(const Dev& D) {
+  WList L;
+  L.vert = D.wide_vert;
+  L.base = D.wide_base;
+  L.slots = static_cast<int>(D.wide_packed >> 32);
+  L.chunks = static_cast<int>(D.wide_packed & 0xffffffffu);
+  return L;
+}
+
+__device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
+  const unsigned chunks = static_cast<unsigned>((deg + kChunk - 1) / kChunk);
+  const unsigned long long old = atomicAdd(&D.ctrl->wq[q], (1ull << 32) | chunks);
+  const size_t slot = static_cast<size_t>(old >> 32);
+  D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
+  D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu);
+}
+
+// Chunk c of a wide list -> vertex and arc range [lo, hi).
+__device__ __forceinline__ int chunk_range(const Dev& D, const WList& L, int c, int* lo, int* hi) {
+  int a = 0, b = L.slots - 1;
+  while (a < b) {
+    const int mid = (a + b + 1) >> 1;
+    if (L.base[mid] <= c) a = mid;
+    else b = mid - 1;
+  }
+  const int v = L.vert[a];
+  const int beg = D.row[v];
+  *lo = beg + (c - L.base[a]) * kChunk;
+  *hi = min(*lo + kChunk, D.row[v + 1]);
+  return v;
+}
+
+// =================================================================== phases
+
+// K1 staticMaxFlow_kernel_1 (code/dyn_data.h:25-35), arc-parallel.
+__device__ void ph_init_residual(const Dev& D, const Range& r) {
+  for (int e = r.gid; e < D.m; e += r.gsz) {
+    D.res[e] = D.cap[e];
+    D.rres[e] = D.cap[D.rev[e]];
+  }
+  for (int v = r.gid; v < D.n; v += r.gsz) D.excess[v] = 0;
+}
+
+// K2 staticMaxFlow_kernel_7 (code/dyn_data.h:39-56), arc-parallel over s's
+// adjacency instead of one thread.  Each arc e of s has its own reverse arc, so
+// only the excess updates need atomics.
+__device__ void ph_saturate_source(const Dev& D, const Range& r, Tally& t) {
+  const int s = D.s;
+  const int b = D.row[s], deg = D.row[s + 1] - b;
+  long long sent = 0;
+  for (int i = r.gid - lane_id(); i < deg; i += r.gsz) {  // wave-uniform loop
+    const int j = i + lane_id();
+    if (j < deg) {
+      const int e = b + j;
+      const int d = D.res[e];
+      if (d > 0) {
+        const int w = D.col[e];
+        add64(&D.excess[w], d);
+        D.res[e] = 0;
+        D.rres[e] += d;
+        const int p = D.rev[e];
+        D.res[p] += d;
+        D.rres[p] -= d;
+        sent += d;
+        t.atom += 1;
+      }
+      t.arc += 1;
+    }
+  }
+  sent = wave_sum_i64(sent);
+  if (lane_id() == 0 && sent) add64(&D.excess[s], -sent);
+}
+
+// Second half of the push-pull static saturation (code/dyn_pp.h:56-70): pull
+// every residual arc into t.
+__device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
+  const int v = D.t;
+  const int b = D.row[v], deg = D.row[v + 1] - b;
+  long long got = 0;
+  for (int i = r.gid - lane_id(); i < deg; i += r.gsz) {
+    const int j = i + lane_id();
+    if (j < deg) {
+      const int e = b + j;
+      const int d = D.rres[e];
+      if (d > 0) {
+        const int w = D.col[e];
+        add64(&D.excess[w], -static_cast<long long>(d));
+        D.res[e] += d;
+        D.rres[e] -= d;
+        const int p = D.rev[e];
+        D.res[p] -= d;
+        D.rres[p] += d;
+        got += d;
+        t.atom += 1;
+      }
+      t.arc += 1;
+    }
+  }
+  got = wave_sum_i64(got);
+  if (lane_id() == 0 && got) add64(&D.excess[v], got);
+}
+
+// ----------------------------------------------------------- global relabel
+// Root predicates of the three global relabels.
+enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2 };
+
+__device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1] - D.row[v]; }
+
+// my_init_1 (code/dyn_data.h:60-67) / staticMaxFlow_kernel_10 (code/dyn_topo.h:60-71)
+// / pp_bfs_init (code/dyn_pp.h:76-83): every vertex unlabelled (height n),
+// roots labelled 0 and appended to order[] (wide roots also to wide queue 0).
+// ctrl->qcnt[0] and ctrl->wq[0] must be 0.
+__device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
+  const int n = D.n;
+  for (int base = r.gid - lane_id(); base < n; base += r.gsz) {
+    const int v = base + lane_id();
+    bool root = false;
+    if (v < n) {
+      const long long ex = D.excess[v];
+      if (mode == BFS_SINK) {
+        root = (v == D.t) || (v != D.s && ex < 0);
+      } else {
+        const bool inB = D.side[v] != 0;
+        root = (v == D.s) || (v == D.t) || (inB && ex < 0) || (!inB && ex > 0);
+      }
+      D.height[v] = root ? 0 : n;
+      t.vert += 1;
+      if (root) {
+        const int d = degree(D, v);
+        if (d > kWideT) wq_append(D, 0, v, d);
+      }
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
+    if (root) D.order[off + rank] = v;
+  }
+}
+
+// final_bfs_init (code/dyn_pp.h:86-93): roots are the part's deficits.
+__device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& t) {
+  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+    const int i = base + lane_id();
+    bool root = false;
+    int v = -1;
+    if (i < cnt) {
+      v = D.part[i];
+      root = D.excess[v] < 0;
+      D.height[v] = root ? 0 : D.n;
+      t.vert += 1;
+      if (root) {
+        const int d = degree(D, v);
+        if (d > kWideT) wq_append(D, 0, v, d);
+      }
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
+    if (root) D.order[off + rank] = v;
+  }
+}
+
+// Whether BFS level expansion may label w through arc e of frontier vertex v.
+__device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int e, int w) {
+  if (mode == BFS_SINK) return D.rres[e] > 0;
+  if (mode == BFS_PP) {
+    if (w == D.s || w == D.t) return false;
+    return vB ? (D.rres[e] > 0) : (D.res[e] > 0);
+  }
+  // BFS_FINAL: non-B neighbours of the stranded part only
+  if (D.side[w] || !D.pfinal[w]) return false;
+  return D.rres[e] > 0;
+}
+
+// Wave walks arcs [lo, hi) of one frontier vertex; newly labelled vertices go
+// to out[] and, when wide, to wide queue qnext.
+__device__ __forceinline__ void bfs_wave_arcs(const Dev& D, int mode, bool vB, int lo, int hi, int nb,
+                                              int32_t* out, int32_t* next_cnt, int qnext, Tally& t) {
+  const int n = D.n;
+  for (int j0 = lo; j0 < hi; j0 += kWave) {
+    const int e = j0 + lane_id();
+    bool hit = false;
+    int w = -1;
+    if (e < hi) {
+      w = D.col[e];
+      t.arc += 1;
+      if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
+        hit = atomicCAS(&D.height[w], n, nb) == n;
+        t.atom += 1;
+      }
+    }
+    int rank;
+    const int off = wave_reserve(next_cnt, hit, &rank);
+    if (hit) {
+      out[off + rank] = w;
+      const int d = degree(D, w);
+      if (d > kWideT) wq_append(D, qnext, w, d);
+    }
+  }
+}
+
+// One BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
+// code/dyn_pp.h:99-120; final_bfs, code/dyn_pp.h:123-140).  Frontier is
+// order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
+// get height lvl+1 (one winner per vertex through compare-and-swap) and are
+// appended after the frontier.
+__device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
+                             int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
+  const int n = D.n;
+  const int nb = lvl + 1;
+  int32_t* out = D.order + lo + cnt;
+  // wide frontier vertices: chunks spread over every wave of the grid
+  for (int c = r.wid; c < W.chunks; c += r.nw) {
+    int a, b;
+    const int v = chunk_range(D, W, c, &a, &b);
+    const bool vB = mode == BFS_PP && D.side[v] != 0;
+    bfs_wave_arcs(D, mode, vB, a, b, nb, out, next_cnt, qnext, t);
+  }
+  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
+    const int i = base + lane_id();
+    int v = -1, b = 0, deg = 0;
+    bool vB = false;
+    if (i < cnt) {
+      v = D.order[lo + i];
+      b = D.row[v];
+      deg = D.row[v + 1] - b;
+      if (deg > kWideT) deg = 0;  // handled through its chunks
+      vB = mode == BFS_PP && D.side[v] != 0;
+      t.vert += 1;
+    }
+    // mid-degree vertices: the whole wave walks the adjacency
+    unsigned long long big = __ballot(deg > kLaneMax);
+    while (big) {
+      const int leader = __ffsll(static_cast<long long>(big)) - 1;
+      big &= big - 1;
+      const int bb = __shfl(b, leader);
+      const int bd = __shfl(deg, leader);
+      const bool bB = __shfl(static_cast<int>(vB), leader) != 0;
+      bfs_wave_arcs(D, mode, bB, bb, bb + bd, nb, out, next_cnt, qnext, t);
+    }
+    // narrow vertices: one lane each, lanes step through arcs in lockstep
+    const int my_deg = (deg <= kLaneMax) ? deg : 0;
+    int maxd = my_deg;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
+    for (int j = 0; j < maxd; ++j) {
+      bool hit = false;
+      int w = -1;
+      if (j < my_deg) {
+        const int e = b + j;
+        w = D.col[e];
+        t.arc += 1;
+        if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
+          hit = atomicCAS(&D.height[w], n, nb) == n;
+          t.atom += 1;
+        }
+      }
+      int rank;
+      const int off = wave_reserve(next_cnt, hit, &rank);
+      if (hit) {
+        out[off + rank] = w;
+        const int d = degree(D, w);
+        if (d > kWideT) wq_append(D, qnext, w, d);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- worklists
+// create_wl (code/dyn_data.h:217-238) over the labelled vertices order[0,cnt):
+// excess > 0, v not s or t, height < n.  Narrow and mid-degree vertices go to
+// wl, wide ones to wide queue kQActive.
+__device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
+  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+    const int i = base + lane_id();
+    bool act = false;
+    int v = -1;
+    if (i < cnt) {
+      v = D.order[i];
+      act = v != D.s && v != D.t && D.excess[v] > 0 && D.height[v] < D.n;
+      t.vert += 1;
+      if (act) {
+        const int d = degree(D, v);
+        if (d > kWideT) {
+          wq_append(D, kQActive, v, d);
+          act = false;
+        }
+      }
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->wl_cnt, act, &rank);
+    if (act) D.wl[off + rank] = v;
+  }
+}
+
+// create_pp_wl (code/dyn_pp.h:499-545): B vertices with excess > 0 to wl,
+// A vertices with excess < 0 to wl2 (wide ones to their wide queues).
+__device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
+  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+    const int i = base + lane_id();
+    bool a = false, b = false;
+    int v = -1;
+    if (i < cnt) {
+      v = D.order[i];
+      if (v != D.s && v != D.t && D.height[v] < D.n) {
+        const long long ex = D.excess[v];
+        if (D.side[v]) a = ex > 0;
+        else b = ex < 0;
+      }
+      t.vert += 1;
+      if (a || b) {
+        const int d = degree(D, v);
+        if (d > kWideT) {
+          wq_append(D, a ? kQActive : kQPull, v, d);
+          a = b = false;
+        }
+      }
+    }
+    int rank;
+    int off = wave_reserve(&D.ctrl->wl_cnt, a, &rank);
+    if (a) D.wl[off + rank] = v;
+    off = wave_reserve(&D.ctrl->wl2_cnt, b, &rank);
+    if (b) D.wl2[off + rank] = v;
+  }
+}
+
+// create_final_wl (code/dyn_pp.h:619-642) over the part list.
+__device__ void ph_build_final_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
+  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+    const int i = base + lane_id();
+    bool act = false;
+    int v = -1;
+    if (i < cnt) {
+      v = D.part[i];
+      act = D.excess[v] > 0 && D.height[v] < D.n;
+      t.vert += 1;
+      if (act) {
+        const int d = degree(D, v);
+        if (d > kWideT) {
+          wq_append(D, kQActive, v, d);
+          act = false;
+        }
+      }
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->wl_cnt, act, &rank);
+    if (act) D.wl[off + rank] = v;
+  }
+}
+
+// ---------------------------------------------------------- push / pull
+// Neighbour filters: all, B only (pp push side), A only (pp pull side).
+enum PushMode { PM_ALL = 0, PM_ONLY_B = 1, PM_ONLY_A = 2 };
+
+__device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
+  if (pmode == PM_ONLY_B) return D.side[w] != 0;
+  if (pmode == PM_ONLY_A) return D.side[w] == 0;
+  return true;
+}
+
+// One push-or-relabel step of staticMaxFlow_kernel_14 (code/dyn_data.h:94-129)
+// for vertex v, given the chosen arc.  Executed by one lane.
+__device__ __forceinline__ void do_push_or_relabel(const Dev& D, int v, long long ex1, int hv,
+                                                   int hh, int fe, Tally& t) {
+  if (fe < 0) return;
+  if (hv > hh) {
+    const int fec = ldr(&D.res[fe]);
+    const int d = static_cast<int>(ex1 < fec ? ex1 : fec);
+    if (d <= 0) return;
+    const int w = D.col[fe];
+    const int p = D.rev[fe];
+    add64(&D.excess[v], -static_cast<long long>(d));
+    add64(&D.excess[w], d);
+    atomicSub(&D.res[fe], d);
+    atomicAdd(&D.rres[fe], d);
+    atomicAdd(&D.res[p], d);
+    atomicSub(&D.rres[p], d);
+    t.push += 1;
+    t.atom += 6;
+  } else {
+    str(&D.height[v], hh + 1);
+    t.relabel += 1;
+  }
+}
+
+// pull_kernel step (code/dyn_pp.h:243-277): a deficit vertex pulls
+// min(-excess, residual) from its chosen neighbour, or relabels.
+__device__ __forceinline__ void do_pull_or_relabel(const Dev& D, int v, long long need, int hv,
+                                                   int hh, int fe, Tally& t) {
+  if (fe < 0) return;
+  if (hv > hh) {
+    const int rec = ldr(&D.rres[fe]);
+    const int d = static_cast<int>(need < rec ? need : rec);
+    if (d <= 0) return;
+    const int w = D.col[fe];
+    const int p = D.rev[fe];
+    add64(&D.excess[w], -static_cast<long long>(d));
+    add64(&D.excess[v], d);
+    atomicSub(&D.rres[fe], d);
+    atomicAdd(&D.res[fe], d);
+    atomicAdd(&D.rres[p], d);
+    atomicSub(&D.res[p], d);
+    t.push += 1;
+    t.atom += 6;
+  } else {
+    str(&D.height[v], hh + 1);
+    t.relabel += 1;
+  }
+}
+
+// Lane path (degree <= kLaneMax): the reference's step exactly — scan for the
+// lowest-height neighbour over residual arcs (first minimum in CSR order),
+// push to it if lower, else relabel — up to kc times.
+template <bool PULL>
+__device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t) {
+  const int n = D.n;
+  for (int c = 0; c < kc; ++c) {
+    const long long ex = ldr64(&D.excess[v]);
+    const int hv = ldr(&D.height[v]);
+    const long long amt = PULL ? -ex : ex;
+    if (!(amt > 0 && (PULL || hv < n))) break;
+    int hh = INT_MAX, fe = -1;
+    for (int e = b; e < b + deg; ++e) {
+      const int w = D.col[e];
+      if (!push_filter(D, pmode, w)) continue;
+      const int h1 = D.height[w];
+      const int resid = PULL ? ldr(&D.rres[e]) : ldr(&D.res[e]);
+      if (h1 < hh && resid > 0) {
+        hh = h1;
+        fe = e;
+      }
+    }
+    t.arc += deg;
+    if (PULL) do_pull_or_relabel(D, v, amt, hv, hh, fe, t);
+    else do_push_or_relabel(D, v, amt, hv, hh, fe, t);
+  }
+}
+
+// Wave discharge of v over arcs [lo, hi): every residual arc to a lower
+// neighbour (the only arcs the reference's step may push along) receives flow
+// in CSR order until the budget runs out; the budget is claimed from excess[v]
+// per 64-arc tile with one atomic, so the chunks of one wide vertex can run in
+// different waves at once.  Push moves flow v->w; pull (push-pull's deficit
+// side) moves flow w->v.  Returns true when the budget ran out; *min_other
+// receives the lowest height among residual arcs that were not lower.
+template <bool PULL>
+__device__ bool wave_discharge(const Dev& D, int v, int hv, int lo, int hi, int pmode,
+                               int* min_other, Tally& t) {
+  int mo = INT_MAX;
+  bool out = false;
+  for (int j0 = lo; j0 < hi; j0 += kWave) {
+    const int e = j0 + lane_id();
+    long long want = 0;
+    int w = -1;
+    if (e < hi) {
+      w = D.col[e];
+      t.arc += 1;
+      if (push_filter(D, pmode, w)) {
+        const int resid = PULL ? ldr(&D.rres[e]) : ldr(&D.res[e]);
+        if (resid > 0) {
+          const int hw = D.height[w];
+          if (hw < hv) want = resid;
+          else mo = min(mo, hw);
+        }
+      }
+    }
+    const long long incl = wave_incl_scan_i64(want);
+    const long long total = __shfl(incl, kWave - 1);
+    if (total == 0) continue;
+    long long avail = 0;
+    if (lane_id() == 0) {
+      // claim up to `total` units of excess (pull: of deficit)
+      const long long old = add64(&D.excess[v], PULL ? total : -total);
+      const long long have = PULL ? -old : old;
+      avail = have <= 0 ? 0 : (have < total ? have : total);
+      if (avail < total) add64(&D.excess[v], PULL ? -(total - avail) : (total - avail));
+      t.atom += avail < total ? 2 : 1;
+    }
+    avail = __shfl(avail, 0);
+    const long long before = incl - want;
+    long long d = avail - before;
+    d = d < 0 ? 0 : (d > want ? want : d);
+    if (d > 0) {
+      const int di = static_cast<int>(d);
+      const int p = D.rev[e];
+      if (PULL) {
+        atomicSub(&D.rres[e], di);
+        atomicAdd(&D.res[e], di);
+        atomicAdd(&D.rres[p], di);
+        atomicSub(&D.res[p], di);
+        add64(&D.excess[w], -d);
+      } else {
+        atomicSub(&D.res[e], di);
+        atomicAdd(&D.rres[e], di);
+        atomicAdd(&D.res[p], di);
+        atomicSub(&D.rres[p], di);
+        add64(&D.excess[w], d);
+      }
+      t.push += 1;
+      t.atom += 5;
+    }
+    if (avail < total) {
+      out = true;
+      break;
+    }
+  }
+  *min_other = wave_min_i32(mo);
+  return out;
+}
+
+// Wave path for mid-degree vertices: up to kc discharges over the whole
+// adjacency, relabelling to (lowest residual neighbour)+1 when a discharge
+// leaves excess behind — the reference's relabel rule.
+template <bool PULL>
+__device__ void wave_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t) {
+  const int n = D.n;
+  for (int c = 0; c < kc; ++c) {
+    long long ex = 0;
+    int hv = 0;
+    if (lane_id() == 0) {
+      ex = ldr64(&D.excess[v]);
+      hv = ldr(&D.height[v]);
+    }
+    ex = __shfl(ex, 0);
+    hv = __shfl(hv, 0);
+    const long long amt = PULL ? -ex : ex;
+    if (!(amt > 0 && (PULL || hv < n))) break;
+    int mo;
+    const bool spent = wave_discharge<PULL>(D, v, hv, b, b + deg, pmode, &mo, t);
+    if (spent) continue;  // re-check: pushes from neighbours may have refilled it
+    if (mo == INT_MAX) break;  // no residual arc at all
+    if (lane_id() == 0) {
+      const long long now = ldr64(&D.excess[v]);
+      if ((PULL ? -now : now) > 0) {
+        str(&D.height[v], mo + 1);
+        t.relabel += 1;
+      }
+    }
+  }
+}
+
+// K6 push/relabel (code/dyn_data.h:89-131; pp_push_kernel code/dyn_pp.h:191-232;
+// pull_kernel code/dyn_pp.h:237-280; push_kernel code/dyn_pp.h:145-185; topo
+// code/dyn_topo.h:90-136).  list == nullptr means every vertex (topology-driven):
+// inactive vertices, s and t are skipped and any active vertex sets *flag.
+// Wide vertices come through W (one discharge pass over each chunk).
+template <bool PULL>
+__device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int cnt, int kc, int pmode,
+                        int32_t* flag, const WList& W, Tally& t) {
+  const int n = D.n;
+  for (int c = r.wid; c < W.chunks; c += r.nw) {
+    int a, b;
+    const int v = chunk_range(D, W, c, &a, &b);
+    long long ex = 0;
+    int hv = 0;
+    if (lane_id() == 0) {
+      ex = ldr64(&D.excess[v]);
+      hv = ldr(&D.height[v]);
+    }
+    ex = __shfl(ex, 0);
+    hv = __shfl(hv, 0);
+    const long long amt = PULL ? -ex : ex;
+    bool act = amt > 0 && (PULL || hv < n);
+    if (!list) act = act && v != D.s && v != D.t;
+    if (!act) continue;
+    if (!list && lane_id() == 0 && *flag == 0) *flag = 1;
+    int mo;
+    wave_discharge<PULL>(D, v, hv, a, b, pmode, &mo, t);
+  }
+  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
+    const int i = base + lane_id();
+    int v = -1, b = 0, deg = 0;
+    if (i < cnt) {
+      v = list ? list[i] : i;
+      b = D.row[v];
+      deg = D.row[v + 1] - b;
+      if (!list) {
+        const bool act = deg <= kWideT && v != D.s && v != D.t && D.excess[v] > 0 && D.height[v] < n;
+        if (!act) v = -1;
+        else if (*flag == 0) *flag = 1;
+      }
+      if (v >= 0) t.vert += 1;
+    }
+    unsigned long long big = __ballot(v >= 0 && deg > kLaneMax);
+    while (big) {
+      const int leader = __ffsll(static_cast<long long>(big)) - 1;
+      big &= big - 1;
+      const int bv = __shfl(v, leader);
+      const int bb = __shfl(b, leader);
+      const int bd = __shfl(deg, leader);
+      wave_push_cycles<PULL>(D, bv, bb, bd, kc, pmode, t);
+    }
+    if (v >= 0 && deg <= kLaneMax) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
+  }
+}
+
+// --------------------------------------------------------------- repair
+// Label repair on one arc (staticMaxFlow_kernel_17, code/dyn_data.h:140-150):
+// residual w->v exists but h(w) > h(v)+1, so saturate w->v.  Heights are not
+// written during this phase, so at most one endpoint of an arc pair acts.
+__device__ __forceinline__ void repair_arc(const Dev& D, int v, int h, int e, int w, Tally& t) {
+  const int d = D.rres[e];
+  if (d > 0 && D.height[w] > h + 1) {
+    D.rres[e] = 0;
+    D.res[e] += d;
+    const int p = D.rev[e];
+    D.rres[p] += d;
+    D.res[p] -= d;
+    add64(&D.excess[w], -static_cast<long long>(d));
+    add64(&D.excess[v], d);
+    t.repair += 1;
+    t.atom += 2;
+  }
+}
+
+// remove_pull (code/dyn_pp.h:381-403): residual v->w exists but h(w) > h(v)+1,
+// so saturate v->w.
+__device__ __forceinline__ void repair_pull_arc(const Dev& D, int v, int h, int e, int w,
+                                                Tally& t) {
+  const int d = D.res[e];
+  if (d > 0 && D.height[w] > h + 1) {
+    D.rres[e] += d;
+    D.res[e] = 0;
+    const int p = D.rev[e];
+    D.rres[p] -= d;
+    D.res[p] += d;
+    add64(&D.excess[v], -static_cast<long long>(d));
+    add64(&D.excess[w], d);
+    t.repair += 1;
+    t.atom += 2;
+  }
+}
+
+// Repair filters: all neighbours, B neighbours (pp_remove_push), A neighbours
+// with the pull rule (remove_pull), part neighbours (remove_push_final).
+enum RepairMode { RM_ALL = 0, RM_B = 1, RM_PULL_A = 2, RM_PFINAL = 3 };
+
+__device__ __forceinline__ void repair_one(const Dev& D, int rmode, int v, int h, int e, Tally& t) {
+  const int w = D.col[e];
+  t.arc += 1;
+  if (rmode == RM_B && !D.side[w]) return;
+  if (rmode == RM_PULL_A && D.side[w]) return;
+  if (rmode == RM_PFINAL && !D.pfinal[w]) return;
+  if (rmode == RM_PULL_A) repair_pull_arc(D, v, h, e, w, t);
+  else repair_arc(D, v, h, e, w, t);
+}
+
+__device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int cnt, int rmode,
+                          const WList& W, Tally& t) {
+  for (int c = r.wid; c < W.chunks; c += r.nw) {
+    int a, b;
+    const int v = chunk_range(D, W, c, &a, &b);
+    const int h = D.height[v];
+    for (int e = a + lane_id(); e < b; e += kWave) repair_one(D, rmode, v, h, e, t);
+  }
+  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
+    const int i = base + lane_id();
+    int v = -1, b = 0, deg = 0, h = 0;
+    if (i < cnt) {
+      v = list ? list[i] : i;
+      b = D.row[v];
+      deg = D.row[v + 1] - b;
+      if (deg > kWideT) v = -1;  // wide: through W
+      else {
+        h = D.height[v];
+        t.vert += 1;
+      }
+    }
+    unsigned long long big = __ballot(v >= 0 && deg > kLaneMax);
+    while (big) {
+      const int leader = __ffsll(static_cast<long long>(big)) - 1;
+      big &= big - 1;
+      const int bv = __shfl(v, leader);
+      const int bb = __shfl(b, leader);
+      const int bd = __shfl(deg, leader);
+      const int bh = __shfl(h, leader);
+      for (int e = bb + lane_id(); e < bb + bd; e += kWave) repair_one(D, rmode, bv, bh, e, t);
+    }
+    if (v >= 0 && deg <= kLaneMax)
+      for (int e = b; e < b + deg; ++e) repair_one(D, rmode, v, h, e, t);
+  }
+}
+
+// ------------------------------------------------------- push-pull phases
+// pp incremental_kernel_20 (code/dyn_pp.h:405-436): clamp, saturate the A->B
+// direction of arcs crossing the stored cut, recompute excess = net inflow.
+// Wide vertices accumulate their chunks' sums atomically; their excess was
+// zeroed in the preceding phase (ph_upd_apply with zero_wide).
+__device__ __forceinline__ long long pp_arc_fix(const Dev& D, int e, uint8_t sv) {
+  int rs = D.res[e], rr = D.rres[e];
+  if (rs < 0) {
+    rr += rs;
+    rs = 0;
+  }
+  if (rr < 0) {
+    rs += rr;
+    rr = 0;
+  }
+  if (sv != D.side[D.col[e]]) {
+    if (sv) {
+      rs += rr;
+      rr = 0;
+    } else {
+      rr += rs;
+      rs = 0;
+    }
+  }
+  D.res[e] = rs;
+  D.rres[e] = rr;
+  return static_cast<long long>(rs) - D.cap[e];
+}
+
+__device__ void ph_pp_cut(const Dev& D, const Range& r, Tally& t) {
+  const int n = D.n;
+  const WList W = wlist_static(D);
+  for (int c = r.wid; c < W.chunks; c += r.nw) {
+    int a, b;
+    const int v = chunk_range(D, W, c, &a, &b);
+    const uint8_t sv = D.side[v];
+    long long acc = 0;
+    for (int e = a + lane_id(); e < b; e += kWave) {
+      acc += pp_arc_fix(D, e, sv);
+      t.arc += 1;
+    }
+    acc = wave_sum_i64(acc);
+    if (lane_id() == 0 && acc) add64(&D.excess[v], acc);
+  }
+  for (int base = r.wid * kWave; base < n; base += r.nw * kWave) {
+    const int v = base + lane_id();
+    int b = 0, deg = 0;
+    uint8_t self = 0;
+    bool mine = false;
+    if (v < n) {
+      b = D.row[v];
+      deg = D.row[v + 1] - b;
+      self = D.side[v];
+      mine = deg <= kWideT;
+      t.vert += 1;
+    }
+    unsigned long long big = __ballot(mine && deg > kLaneMax);
+    while (big) {
+      const int leader = __ffsll(static_cast<long long>(big)) - 1;
+      big &= big - 1;
+      const int bv = __shfl(v, leader);
+      const int bb = __shfl(b, leader);
+      const int bd = __shfl(deg, leader);
+      const uint8_t bs = static_cast<uint8_t>(__shfl(static_cast<int>(self), leader));
+      long long acc = 0;
+      for (int e = bb + lane_id(); e < bb + bd; e += kWave) {
+        acc += pp_arc_fix(D, e, bs);
+        t.arc += 1;
+      }
+      acc = wave_sum_i64(acc);
+      if (lane_id() == 0) D.excess[bv] = acc;
+    }
+    if (mine && deg <= kLaneMax) {
+      long long acc = 0;
+      for (int e = b; e < b + deg; ++e) acc += pp_arc_fix(D, e, self);
+      t.arc += deg;
+      D.excess[v] = acc;
+    }
+  }
+}
+
+// find_cut (code/dyn_pp.h:440-449): B = labelled by the last relabel.
+__device__ void ph_find_cut(const Dev& D, const Range& r) {
+  for (int v = r.gid; v < D.n; v += r.gsz) D.side[v] = D.height[v] < D.n ? 1 : 0;
+}
+
+// find_part (code/dyn_pp.h:330-356): vertices the last two-sided relabel left
+// unlabelled move to A and form the part list; every height is reset.
+__device__ void ph_find_part(const Dev& D, const Range& r, Tally& t) {
+  const int n = D.n;
+  for (int base = r.gid - lane_id(); base < n; base += r.gsz) {
+    const int v = base + lane_id();
+    bool in = false;
+    if (v < n) {
+      in = D.height[v] == n;
+      D.pfinal[v] = in ? 1 : 0;
+      if (in) D.side[v] = 0;
+      D.height[v] = n;
+      t.vert += 1;
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->part_cnt, in, &rank);
+    if (in) D.part[off + rank] = v;
+  }
+}
+
+// find_final_cut (code/dyn_pp.h:452-460).
+__device__ void ph_find_final_cut(const Dev& D, const Range& r) {
+  for (int v = r.gid; v < D.n; v += r.gsz)
+    if (!D.side[v] && D.height[v] < D.n) D.side[v] = 1;
+}
+
+// recalculate_max_flow_kernel_39 (code/dyn_data.h:208-215) as a reduction that
+// leaves excess untouched: flow = excess[t] + sum of excess[v] < 0, v != s.
+__device__ void ph_flow(const Dev& D, const Range& r) {
+  long long acc = 0;
+  for (int v = r.gid; v < D.n; v += r.gsz) {
+    const long long ex = D.excess[v];
+    if (v == D.t) acc += ex;
+    if (ex < 0 && v != D.s) acc += ex;
+  }
+  acc = wave_sum_i64(acc);
+  if (lane_id() == 0 && acc) add64(&D.ctrl->flow, acc);
+}
+
+// ---------------------------------------------------------- update phases
+__device__ __forceinline__ unsigned long long hash_key(unsigned long long k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+__device__ __forceinline__ unsigned long long pair_key(int u, int v) {
+  return (static_cast<unsigned long long>(static_cast<unsigned>(u)) << 32) |
+         static_cast<unsigned>(v);
+}
+
+__device__ int lookup_arc(const Dev& D, int u, int v) {
+  const unsigned long long key = pair_key(u, v);
+  unsigned long long h = hash_key(key) & D.hmask;
+  for (;;) {
+    const unsigned long long k = D.hkey[h];
+    if (k == key) return D.hval[h];
+    if (k == kEmptyKey) return -1;
+    h = (h + 1) & D.hmask;
+  }
+}
+
+// OnAdd_kernel arc search (code/dyn_data.h:191-201) by hash lookup, plus
+// last-record-wins resolution of duplicate arcs within the batch.
+__device__ void ph_upd_lookup(const Dev& D, const Range& r) {
+  int missing = 0, invalid = 0;
+  for (int i = r.gid; i < D.k; i += r.gsz) {
+    const int a = D.usrc[i], b = D.udst[i];
+    int arc = -2;
+    if (a < 0 || a >= D.n || b < 0 || b >= D.n) {
+      ++invalid;
+    } else if (a != b) {  // self-loops skipped (code/update.hpp:82, code/dyn_data.h:194)
+      arc = lookup_arc(D, a, b);
+      if (arc < 0) ++missing;
+      else atomicMax(&D.owner[arc], i);
+    }
+    D.uarc[i] = arc;
+  }
+  if (missing) atomicAdd(&D.ctrl->missing, missing);
+  if (invalid) atomicAdd(&D.ctrl->invalid, invalid);
+}
+
+// OnAdd_kernel update (code/dyn_data.h:202-205) for the winning record of each
+// arc: residual and its mirror move with the capacity.  capacity_only mirrors
+// update_kernel (code/dyn_data_bench.cu:300-322).  zero_wide clears the excess
+// of every wide vertex for the push-pull recompute that follows.
+__device__ void ph_upd_apply(const Dev& D, const Range& r, bool capacity_only, bool zero_wide) {
+  if (D.ctrl->missing || D.ctrl->invalid) return;
+  for (int i = r.gid; i < D.k; i += r.gsz) {
+    const int arc = D.uarc[i];
+    if (arc < 0 || D.owner[arc] != i) continue;
+    const int nw = D.uw[i];
+    if (!capacity_only) {
+      const int delta = nw - D.cap[arc];
+      D.res[arc] += delta;
+      D.rres[D.rev[arc]] += delta;
+    }
+    D.cap[arc] = nw;
+  }
+  if (zero_wide) {
+    const int slots = static_cast<int>(D.wide_packed >> 32);
+    for (int i = r.gid; i < slots; i += r.gsz) D.excess[D.wide_vert[i]] = 0;
+  }
+}
+
+// Decrement repair restricted to the updated arcs (the only ones whose
+// residual can be negative): incremental_kernel_20 (code/dyn_data.h:157-172)
+// cancels the flow above the new capacity, and the net-inflow recompute of
+// incremental_kernel_22 (code/dyn_data.h:173-183) reduces to moving that
+// amount between the two endpoint excesses.
+__device__ void ph_upd_clamp(const Dev& D, const Range& r, Tally& t) {
+  if (D.ctrl->missing || D.ctrl->invalid) return;
+  for (int i = r.gid; i < D.k; i += r.gsz) {
+    const int arc = D.uarc[i];
+    if (arc < 0 || D.owner[arc] != i) continue;
+    const int rs = D.res[arc];
+    t.arc += 1;
+    if (rs < 0) {
+      const int a = -rs;
+      const int p = D.rev[arc];
+      D.res[arc] = 0;
+      D.rres[p] = 0;
+      D.rres[arc] -= a;
+      D.res[p] -= a;
+      add64(&D.excess[D.usrc[i]], a);
+      add64(&D.excess[D.udst[i]], -static_cast<long long>(a));
+      t.atom += 2;
+    }
+  }
+}
+
+__device__ void ph_upd_release(const Dev& D, const Range& r) {
+  for (int i = r.gid; i < D.k; i += r.gsz) {
+    const int arc = D.uarc[i];
+    if (arc >= 0) D.owner[arc] = -1;
+  }
+}
+
+// Reachability from s and from positive-excess vertices over res > 0 arcs
+// (S_min), one BFS level.  mark is a caller scratch array of n ints.
+__device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark) {
+  for (int base = r.gid - lane_id(); base < D.n; base += r.gsz) {
+    const int v = base + lane_id();
+    bool root = false;
+    if (v < D.n) {
+      root = v == D.s || (v != D.t && D.excess[v] > 0);
+      mark[v] = root ? 1 : 0;
+    }
+    int rank;
+    const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
+    if (root) D.order[off + rank] = v;
+  }
+}
+
+__device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int lo, int cnt,
+                             int32_t* next_cnt) {
+  int32_t* out = D.order + lo + cnt;
+  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
+    const int i = base + lane_id();
+    int b = 0, deg = 0;
+    if (i < cnt) {
+      const int v = D.order[lo + i];
+      b = D.row[v];
+      deg = D.row[v + 1] - b;
+    }
+    unsigned long long big = __ballot(deg > kLaneMax);
+    while (big) {
+      const int leader = __ffsll(static_cast<long long>(big)) - 1;
+      big &= big - 1;
+      const int bb = __shfl(b, leader);
+      const int bd = __shfl(deg, leader);
+      for (int j0 = 0; j0 < bd; j0 += kWave) {
+        const int j = j0 + lane_id();
+        bool hit = false;
+        int w = -1;
+        if (j < bd) {
+          const int e = bb + j;
+          w = D.col[e];
+          if (D.res[e] > 0 && mark[w] == 0) hit = atomicCAS(&mark[w], 0, 1) == 0;
+        }
+        int rank;
+        const int off = wave_reserve(next_cnt, hit, &rank);
+        if (hit) out[off + rank] = w;
+      }
+    }
+    const int my_deg = deg <= kLaneMax ? deg : 0;
+    int maxd = my_deg;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
+    for (int j = 0; j < maxd; ++j) {
+      bool hit = false;
+      int w = -1;
+      if (j < my_deg) {
+        const int e = b + j;
+        w = D.col[e];
+        if (D.res[e] > 0 && mark[w] == 0) hit = atomicCAS(&mark[w], 0, 1) == 0;
+      }
+      int rank;
+      const int off = wave_reserve(next_cnt, hit, &rank);
+      if (hit) out[off + rank] = w;
+    }
+  }
+}
+
+// ============================================================ stepped kernels
+__global__ void __launch_bounds__(kBlock) k_init_residual(Dev D) { ph_init_residual(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_saturate_source(Dev D) {
+  Tally t;
+  ph_saturate_source(D, make_range(), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_saturate_sink_pp(Dev D) {
+  Tally t;
+  ph_saturate_sink_pp(D, make_range(), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_bfs_init(Dev D, int mode) {
+  Tally t;
+  ph_bfs_init(D, make_range(), mode, t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_bfs_init_final(Dev D, int cnt) {
+  Tally t;
+  ph_bfs_init_final(D, make_range(), cnt, t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, int cnt, int lvl,
+                                                      unsigned long long wpk) {
+  Tally t;
+  const WList W = wlist(D, lvl % 3, wpk);
+  ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
+  Tally t;
+  ph_build_wl(D, make_range(), cnt, t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_build_pp_wl(Dev D, int cnt) {
+  Tally t;
+  ph_build_pp_wl(D, make_range(), cnt, t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_build_final_wl(Dev D, int cnt) {
+  Tally t;
+  ph_build_final_wl(D, make_range(), cnt, t);
+  t.flush(D.ctrl);
+}
+// which: 0 = all vertices (topology), 1 = wl, 2 = wl2 (pull)
+__global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int kc, int pmode,
+                                                 unsigned long long wpk) {
+  Tally t;
+  const Range r = make_range();
+  if (which == 0) {
+    ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[0], wlist_static(D), t);
+  } else if (which == 1) {
+    ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t);
+  } else {
+    ph_push<true>(D, r, D.wl2, cnt, kc, PM_ONLY_A, nullptr, wlist(D, kQPull, wpk), t);
+  }
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, int rmode,
+                                                   unsigned long long wpk) {
+  Tally t;
+  const Range r = make_range();
+  if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_static(D), t);
+  else if (which == 1) ph_repair(D, r, D.wl, cnt, rmode, wlist(D, kQActive, wpk), t);
+  else ph_repair(D, r, D.wl2, cnt, rmode, wlist(D, kQPull, wpk), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_pp_cut(Dev D) {
+  Tally t;
+  ph_pp_cut(D, make_range(), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_find_cut(Dev D) { ph_find_cut(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_find_part(Dev D) {
+  Tally t;
+  ph_find_part(D, make_range(), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_find_final_cut(Dev D) { ph_find_final_cut(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_flow(Dev D) { ph_flow(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_upd_lookup(Dev D) { ph_upd_lookup(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_upd_apply(Dev D, int capacity_only, int zero_wide) {
+  ph_upd_apply(D, make_range(), capacity_only != 0, zero_wide != 0);
+}
+__global__ void __launch_bounds__(kBlock) k_upd_clamp(Dev D) {
+  Tally t;
+  ph_upd_clamp(D, make_range(), t);
+  t.flush(D.ctrl);
+}
+__global__ void __launch_bounds__(kBlock) k_upd_release(Dev D) { ph_upd_release(D, make_range()); }
+__global__ void __launch_bounds__(kBlock) k_cut_init(Dev D, int32_t* mark) { ph_cut_init(D, make_range(), mark); }
+__global__ void __launch_bounds__(kBlock) k_cut_level(Dev D, int32_t* mark, int lo, int cnt, int slot) {
+  ph_cut_level(D, make_range(), mark, lo, cnt, &D.ctrl->qcnt[slot]);
+}
+
+// Builds the (tail, head) -> arc hash index, one thread per arc; for a repeated
+// (tail, head) pair the highest arc index wins, as the reference's last-match scan.
+__global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < D.m; e += gridDim.x * blockDim.x) {
+    int a = 0, b = D.n - 1;  // tail = largest v with row[v] <= e
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (D.row[mid] <= e) a = mid;
+      else b = mid - 1;
+    }
+    const unsigned long long key = pair_key(a, D.col[e]);
+    unsigned long long h = hash_key(key) & D.hmask;
+    for (;;) {
+      const unsigned long long prev = atomicCAS(&D.hkey[h], kEmptyKey, key);
+      if (prev == kEmptyKey || prev == key) {
+        atomicMax(&D.hval[h], e);
+        break;
+      }
+      h = (h + 1) & D.hmask;
+    }
+  }
+}
+
+// ======================================================== persistent kernel
+// One cooperative launch runs a whole solve: the batch prelude (update
+// lookup/apply/clamp, source re-saturation), the relabel/push/repair rounds of
+// the chosen variant, the push-pull finishing phases and the flow reduction.
+// Control decisions read device counters after a grid barrier, so every block
+// takes the same branch.  A counter is cleared only after a barrier that every
+// reader of it has passed, and before the barrier preceding its next writer.
+
+__device__ __forceinline__ bool is_block0_thread0() { return blockIdx.x == 0 && threadIdx.x == 0; }
+
+// Global relabel with the given root mode; returns the number of vertices
+// labelled (order[0, visited)).  On entry ctrl->qcnt[] and ctrl->wq[0..2] are
+// 0; on exit they are 0 again.
+__device__ int relabel_persistent(const Dev& D, cg::grid_group& G, const Range& r, int mode,
+                                  int part_cnt, Tally& t) {
+  if (mode == BFS_FINAL) ph_bfs_init_final(D, r, part_cnt, t);
+  else ph_bfs_init(D, r, mode, t);
+  G.sync();
+  int lo = 0, lvl = 0;
+  for (;;) {
+    const int cur = lvl % 3;
+    const int cnt = ldr(&D.ctrl->qcnt[cur]);
+    if (cnt == 0) break;
+    const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
+    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+    if (is_block0_thread0()) {
+      // slot (lvl+2)%3 was read during level lvl-1 and is next written during
+      // level lvl+1
+      D.ctrl->qcnt[(lvl + 2) % 3] = 0;
+      D.ctrl->wq[(lvl + 2) % 3] = 0;
+      D.ctrl->ctr[C_LEVELS] += 1;
+    }
+    lo += cnt;
+    ++lvl;
+    G.sync();
+  }
+  G.sync();
+  if (is_block0_thread0()) {
+    D.ctrl->qcnt[0] = D.ctrl->qcnt[1] = D.ctrl->qcnt[2] = 0;
+    D.ctrl->wq[0] = D.ctrl->wq[1] = D.ctrl->wq[2] = 0;
+    D.ctrl->visited = lo;
+    D.ctrl->ctr[C_GRELABEL] += 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
+  cg::grid_group G = cg::this_grid();
+  const Range r = make_range();
+  Tally t;
+  const int n = D.n;
+  const bool pp = C.variant == DMF_VARIANT_PP;
+  const bool topo = C.variant == DMF_VARIANT_TOPO;
+
+  // ---- prelude
+  if (C.entry == 0) {
+    ph_init_residual(D, r);
+    G.sync();
+    ph_saturate_source(D, r, t);
+    G.sync();
+    if (pp) {
+      ph_saturate_sink_pp(D, r, t);
+      G.sync();
+    }
+  } else if (C.entry == 1) {
+    ph_upd_lookup(D, r);
+    G.sync();
+    if (D.ctrl->missing || D.ctrl->invalid) {
+      ph_upd_release(D, r);  // leave the state untouched, owners cleared
+      t.flush(D.ctrl);
+      return;  // uniform: every block read the same counters after the barrier
+    }
+    ph_upd_apply(D, r, false, pp);
+    G.sync();
+    if (pp) {
+      ph_pp_cut(D, r, t);
+    } else {
+      ph_upd_clamp(D, r, t);
+      G.sync();
+      ph_saturate_source(D, r, t);
+    }
+    G.sync();
+    ph_upd_release(D, r);
+    if (is_block0_thread0()) D.ctrl->ctr[C_UPD] += D.k;
+    G.sync();
+  }
+
+  int round = 0;
+  bool conv = true;
+  if (!pp || C.entry == 0) {
+    // ---- data / topo rounds (also the push-pull static solve)
+    int since = C.interval;  // rounds since the last global relabel
+    int visited = 0;
+    for (;;) {
+      bool fresh = false;
+      if (since >= C.interval) {
+        visited = relabel_persistent(D, G, r, BFS_SINK, 0, t);
+        since = 0;
+        fresh = true;
+      }
+      int cnt = 0;
+      unsigned long long wpk = 0;
+      if (!topo) {
+        ph_build_wl(D, r, visited, t);
+        G.sync();
+        cnt = ldr(&D.ctrl->wl_cnt);
+        wpk = ldru64(&D.ctrl->wq[kQActive]);
+        if (cnt == 0 && wpk == 0) {
+          if (fresh) break;
+          since = C.interval;  // confirm convergence with fresh labels
+          G.sync();
+          continue;
+        }
+      }
+      if (round >= C.max_rounds) {
+        conv = false;
+        break;
+      }
+      if (topo) {
+        // flag slots alternate by round: the slot for the next round was last
+        // read one round ago, so clearing it here cannot race with a reader.
+        int32_t* fl = &D.ctrl->flag[round & 1];
+        ph_push<false>(D, r, nullptr, n, C.kc, PM_ALL, fl, wlist_static(D), t);
+        G.sync();
+        const int flag = ldr(fl);
+        if (is_block0_thread0()) D.ctrl->flag[(round + 1) & 1] = 0;
+        ph_repair(D, r, nullptr, n, RM_ALL, wlist_static(D), t);
+        G.sync();
+        if (!flag) break;
+      } else {
+        const WList W = wlist(D, kQActive, wpk);
+        ph_push<false>(D, r, D.wl, cnt, C.kc, PM_ALL, nullptr, W, t);
+        G.sync();
+        // every block read wl_cnt / wq before the push barrier; the next
+        // append comes after the repair barrier
+        if (is_block0_thread0()) {
+          D.ctrl->wl_cnt = 0;
+          D.ctrl->wq[kQActive] = 0;
+        }
+        ph_repair(D, r, D.wl, cnt, RM_ALL, W, t);
+        G.sync();
+      }
+      ++round;
+      ++since;
+    }
+    if (pp && conv) {
+      ph_find_cut(D, r);
+      G.sync();
+    }
+  } else {
+    // ---- push-pull batch: part_pp (code/dyn_pp_bench.cu:6-88)
+    for (;;) {
+      const int visited = relabel_persistent(D, G, r, BFS_PP, 0, t);
+      ph_build_pp_wl(D, r, visited, t);
+      G.sync();
+      const int np = ldr(&D.ctrl->wl_cnt), nq = ldr(&D.ctrl->wl2_cnt);
+      const unsigned long long wp = ldru64(&D.ctrl->wq[kQActive]);
+      const unsigned long long wq = ldru64(&D.ctrl->wq[kQPull]);
+      if (np + nq == 0 && wp == 0 && wq == 0) break;
+      if (round >= C.max_rounds) {
+        conv = false;
+        break;
+      }
+      const WList WP = wlist(D, kQActive, wp), WQ = wlist(D, kQPull, wq);
+      ph_push<false>(D, r, D.wl, np, C.kc, PM_ONLY_B, nullptr, WP, t);
+      ph_push<true>(D, r, D.wl2, nq, C.kc, PM_ONLY_A, nullptr, WQ, t);
+      G.sync();
+      ph_repair(D, r, D.wl, np, RM_B, WP, t);
+      ph_repair(D, r, D.wl2, nq, RM_PULL_A, WQ, t);
+      G.sync();
+      if (is_block0_thread0()) {  // next append follows the relabel's barriers
+        D.ctrl->wl_cnt = D.ctrl->wl2_cnt = 0;
+        D.ctrl->wq[kQActive] = D.ctrl->wq[kQPull] = 0;
+      }
+      ++round;
+    }
+    // ---- part_final (code/dyn_pp_bench.cu:91-152)
+    if (conv) {
+      ph_find_part(D, r, t);
+      G.sync();
+      const int parts = ldr(&D.ctrl->part_cnt);
+      if (parts > 0) {
+        for (;;) {
+          relabel_persistent(D, G, r, BFS_FINAL, parts, t);
+          ph_build_final_wl(D, r, parts, t);
+          G.sync();
+          const int cnt = ldr(&D.ctrl->wl_cnt);
+          const unsigned long long wpk = ldru64(&D.ctrl->wq[kQActive]);
+          if (cnt == 0 && wpk == 0) break;
+          if (round >= C.max_rounds) {
+            conv = false;
+            break;
+          }
+          const WList W = wlist(D, kQActive, wpk);
+          ph_push<false>(D, r, D.wl, cnt, C.kc, PM_ALL, nullptr, W, t);
+          G.sync();
+          ph_repair(D, r, D.wl, cnt, RM_PFINAL, W, t);
+          G.sync();
+          if (is_block0_thread0()) {
+            D.ctrl->wl_cnt = 0;
+            D.ctrl->wq[kQActive] = 0;
+          }
+          ++round;
+        }
+        if (conv) ph_find_final_cut(D, r);
+      }
+      G.sync();
+    }
+  }
+  G.sync();
+  if (is_block0_thread0()) {
+    D.ctrl->wl_cnt = D.ctrl->wl2_cnt = D.ctrl->part_cnt = 0;
+    D.ctrl->wq[kQActive] = D.ctrl->wq[kQPull] = 0;
+    D.ctrl->flag[0] = D.ctrl->flag[1] = 0;
+    D.ctrl->rounds = round;
+    D.ctrl->ctr[C_ROUNDS] += round;
+    D.ctrl->flow = 0;
+    if (!conv) D.ctrl->status = DMF_ENOCONV;
+  }
+  G.sync();
+  ph_flow(D, r);
+  t.flush(D.ctrl);
+}
+
+}  // namespace dmfdev

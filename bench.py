@@ -204,7 +204,8 @@ def main():
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="data")
-    ap.add_argument("--mode", type=int, default=0, help="0 persistent, 1 stepped")
+    ap.add_argument("--mode", type=int, default=1, help="0 persistent one-launch solve, 1 stepped")
+    ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -220,7 +221,7 @@ def main():
         sys.exit(1)
     g, s, t, _, batches = make_workload(args.scale, args.ef, args.seed, 10)
     kc = max(1, g.m // g.n)
-    plain = dmf.default_params(g, variant=args.variant, mode=args.mode)
+    plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
     recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain")
     S = summarize(recs)
     verified = None
@@ -245,7 +246,7 @@ def main():
             "workload": f"rmat-{args.scale} ef{args.ef} seed {args.seed}, 10 successive batches 1%..10% "
                         f"of {g.m // 2 if False else 'input'} edges, 50/50 inc/dec, caps U[1,100]",
             "n": g.n, "m": g.m, "s": s, "t": t,
-            "variant": args.variant, "kc": kc, "relabel_interval": 1,
+            "variant": args.variant, "kc": kc, "relabel_interval": args.interval,
             "driver": "persistent" if args.mode == 0 else "stepped",
         },
         "p99_ms": round(S["p99_ms"], 4),

@@ -28,6 +28,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -137,7 +138,14 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
     if (cnt == 0) break;
     if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) return rc;
     if ((rc = zero_dev(E, &D.ctrl->wq[(lvl + 1) % 3], sizeof(unsigned long long)))) return rc;
-    LAUNCH(k_bfs_level, phase_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk);
+    if (use_bottom_up(D, mode, cnt)) {
+      LAUNCH(k_bfs_level, blocks_for(E->n), D, mode, lo, cnt, lvl, wpk, 1);
+    } else if (use_collect(D, cnt)) {
+      LAUNCH(k_bfs_level, phase_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 2);
+      LAUNCH(k_bfs_collect, blocks_for(E->n), D, lo, cnt, lvl);
+    } else {
+      LAUNCH(k_bfs_level, phase_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 0);
+    }
     lo += cnt;
     ++lvl;
     if (lvl > E->n + 1) return DMF_EDEVICE;
@@ -323,6 +331,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
 // Zero the control block's counters before a solve.
 int begin_solve(dmf_engine* E) {
   HIP_OK(hipMemsetAsync(E->D.ctrl, 0, sizeof(Ctrl), E->stream));
+  HIP_OK(hipMemsetAsync(E->D.ctrs, 0, sizeof(unsigned long long) * C_N * kSlots, E->stream));
   E->last = dmf_counters{};
   E->launches = 0;
   return DMF_OK;
@@ -334,18 +343,27 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
   const Ctrl& c = *E->h_ctrl;
-  E->last.arc_visits = c.ctr[C_ARC];
-  E->last.vertex_visits = c.ctr[C_VERT];
-  E->last.pushes = c.ctr[C_PUSH];
-  E->last.relabels = c.ctr[C_RELABEL];
-  E->last.repairs = c.ctr[C_REPAIR];
-  if (c.ctr[C_LEVELS] || c.ctr[C_GRELABEL]) {  // persistent driver counts on device
-    E->last.bfs_levels = c.ctr[C_LEVELS];
-    E->last.global_relabels = c.ctr[C_GRELABEL];
+  unsigned long long ctr[C_N] = {};
+  {
+    std::vector<unsigned long long> slots(static_cast<size_t>(C_N) * kSlots);
+    HIP_OK(hipMemcpyAsync(slots.data(), E->D.ctrs, sizeof(unsigned long long) * slots.size(),
+                          hipMemcpyDeviceToHost, E->stream));
+    HIP_OK(hipStreamSynchronize(E->stream));
+    for (int i = 0; i < C_N; ++i)
+      for (int j = 0; j < kSlots; ++j) ctr[i] += slots[static_cast<size_t>(i) * kSlots + j];
   }
-  if (c.ctr[C_ROUNDS]) E->last.rounds = c.ctr[C_ROUNDS];
-  E->last.updates = c.ctr[C_UPD] ? c.ctr[C_UPD] : E->last.updates;
-  E->last.atomics = c.ctr[C_ATOMIC];
+  E->last.arc_visits = ctr[C_ARC];
+  E->last.vertex_visits = ctr[C_VERT];
+  E->last.pushes = ctr[C_PUSH];
+  E->last.relabels = ctr[C_RELABEL];
+  E->last.repairs = ctr[C_REPAIR];
+  if (ctr[C_LEVELS] || ctr[C_GRELABEL]) {  // persistent driver counts on device
+    E->last.bfs_levels = ctr[C_LEVELS];
+    E->last.global_relabels = ctr[C_GRELABEL];
+  }
+  if (ctr[C_ROUNDS]) E->last.rounds = ctr[C_ROUNDS];
+  E->last.updates = ctr[C_UPD] ? ctr[C_UPD] : E->last.updates;
+  E->last.atomics = ctr[C_ATOMIC];
   E->last.launches = E->launches;
   E->last.device_ms = ms;
   if (c.missing) return DMF_ENOARC;
@@ -405,7 +423,7 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
   p->relabel_interval = 1;
   p->scheduler = DMF_SCHED_FIXED;
   p->max_rounds = 0;
-  p->mode = 0;
+  p->mode = 1;  // stepped: fastest single-graph driver on MI355X
 }
 
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
@@ -442,6 +460,7 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
   const size_t nwide = std::max<size_t>(1, wide_v.size());
   int32_t *row, *col, *rev, *cap, *res, *rres, *height, *order, *wl, *wl2, *part, *owner, *hval;
   int32_t *wq_vert, *wq_base, *wide_vert, *wide_base;
+  unsigned long long* ctrs;
   long long* excess;
   uint8_t *side, *pfinal;
   Ctrl* ctrl;
@@ -455,7 +474,7 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
       (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &hkey, hcap)) || (rc = dalloc(E, &hval, hcap)) ||
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &wq_vert, nwide * kNumQ)) ||
       (rc = dalloc(E, &wq_base, nwide * kNumQ)) || (rc = dalloc(E, &wide_vert, nwide)) ||
-      (rc = dalloc(E, &wide_base, nwide))) {
+      (rc = dalloc(E, &wide_base, nwide)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots))) {
     dmf_engine_destroy(E);
     return rc;
   }
@@ -464,6 +483,13 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
   D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
   D.hmask = hcap - 1; D.owner = owner;
   D.wq_vert = wq_vert; D.wq_base = wq_base; D.wq_cap = static_cast<int32_t>(nwide);
+  D.ctrs = ctrs;
+  {
+    const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
+    D.bottom_up = bu ? std::atoi(bu) : 0;
+    const char* co = std::getenv("DMF_BFS_COLLECT");
+    D.collect = co ? std::atoi(co) : 0;
+  }
   D.wide_vert = wide_vert; D.wide_base = wide_base;
   D.wide_packed = (static_cast<unsigned long long>(wide_v.size()) << 32) |
                   static_cast<unsigned long long>(wide_chunks);
@@ -515,7 +541,9 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
                                                    kBlock, 0) == hipSuccess &&
       per_cu > 0) {
     E->num_cu = prop.multiProcessorCount;
-    E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, 4);
+    const char* cap = std::getenv("DMF_COOP_BLOCKS_PER_CU");
+    const int want = cap ? std::max(1, std::atoi(cap)) : 8;
+    E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
   }
   *out = E;
   return DMF_OK;
@@ -544,6 +572,7 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(D.hkey);
   hipFree(D.hval);
   hipFree(D.owner);
+  hipFree(D.ctrs);
   hipFree(D.wq_vert);
   hipFree(D.wq_base);
   hipFree(const_cast<int32_t*>(D.wide_vert));

@@ -11,6 +11,9 @@
 //                                 any wave of the grid picks up, so a hub of
 //                                 60k arcs is spread over ~240 waves instead
 //                                 of serialising one wave.
+// The BFS expands the non-wide part of a frontier kGroup vertices per wave
+// task as one flattened arc range (coalesced, balanced to within kGroup*kWideT
+// arcs per task).
 // A wide queue is a pair of arrays (vertex, first chunk) filled through one
 // 64-bit atomic that reserves the slot (high half) and the chunk range (low
 // half) together, so first-chunk values are monotone in slot order and a chunk
@@ -32,8 +35,11 @@ namespace dmfdev {
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 constexpr int kLaneMax = 32;
-constexpr int kWideT = 1024;
+constexpr int kWideT = 256;
 constexpr int kChunk = 256;
+constexpr int kGroup = 16;      // frontier vertices per wave task in the flattened BFS expansion
+constexpr int kSlots = 64;       // work counters are spread over this many addresses
+constexpr int kStageCap = 512;   // per-wave LDS staging of frontier appends
 
 enum CtrIdx {
   C_ARC = 0,
@@ -68,7 +74,6 @@ struct Ctrl {
   int32_t rounds;     // rounds of the last solve
   int32_t pad[3];
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
-  unsigned long long ctr[C_N];
   long long flow;
 };
 
@@ -89,6 +94,9 @@ struct Dev {
   uint8_t* side;
   uint8_t* pfinal;
   Ctrl* ctrl;
+  unsigned long long* ctrs;  // C_N x kSlots work counters
+  int32_t bottom_up;         // frontier size divisor for bottom-up BFS levels (0 = never)
+  int32_t collect;           // frontier size divisor for plain-store + collect levels (0 = never)
   unsigned long long* hkey;
   int32_t* hval;
   unsigned long long hmask;
@@ -180,22 +188,79 @@ __device__ __forceinline__ int wave_reserve(int32_t* counter, bool pred, int* ra
   return base;
 }
 
-// Per-thread work counters, flushed once per phase with one atomic per wave.
+// Per-thread work counters.  flush() reduces them over the block and adds the
+// sums into one of kSlots copies of each counter: one atomic per block and
+// counter, spread over kSlots addresses (same-address atomics serialise).
+// Every thread of the block must call flush().
+__device__ __forceinline__ void ctr_add(unsigned long long* ctrs, int idx, unsigned long long v) {
+  atomicAdd(&ctrs[idx * kSlots + (blockIdx.x % kSlots)], v);
+}
+
 struct Tally {
   unsigned long long arc = 0, vert = 0, push = 0, relabel = 0, repair = 0, atom = 0;
-  __device__ void flush(Ctrl* c) {
+  __device__ void flush(unsigned long long* ctrs) {
+    __shared__ unsigned long long red[6][kBlock / kWave];
     unsigned long long v[6] = {arc, vert, push, relabel, repair, atom};
-    const int idx[6] = {C_ARC, C_VERT, C_PUSH, C_RELABEL, C_REPAIR, C_ATOMIC};
+    const int wv = threadIdx.x / kWave;
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       unsigned long long x = v[i];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-      if (lane_id() == 0 && x) atomicAdd(&c->ctr[idx[i]], x);
+      if (lane_id() == 0) red[i][wv] = x;
     }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int idx[6] = {C_ARC, C_VERT, C_PUSH, C_RELABEL, C_REPAIR, C_ATOMIC};
+      unsigned long long sum = 0;
+      for (int w = 0; w < kBlock / kWave; ++w) sum += red[threadIdx.x][w];
+      if (sum) ctr_add(ctrs, idx[threadIdx.x], sum);
+    }
+    __syncthreads();
     arc = vert = push = relabel = repair = atom = 0;
   }
 };
+
+// Per-wave staging of list appends in LDS: a wave collects up to kStageCap ids
+// and publishes them with one atomic on the shared list counter.  push() and
+// flush() must be reached by all 64 lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct Stage {
+  int32_t* buf;  // this wave's LDS slice, kStageCap entries
+  int cnt;       // wave-uniform fill level
+  __device__ void flush(int32_t* gcnt, int32_t* out) {
+    if (cnt == 0) return;
+    wave_lds_sync();
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(gcnt, cnt);
+    base = __shfl(base, 0);
+    for (int i = lane_id(); i < cnt; i += kWave) out[base + i] = buf[i];
+    wave_lds_sync();
+    cnt = 0;
+  }
+  __device__ void push(bool hit, int v, int32_t* gcnt, int32_t* out) {
+    const unsigned long long b = __ballot(hit);
+    if (b == 0) return;
+    const int rank = __popcll(b & ((1ull << lane_id()) - 1ull));
+    if (hit) buf[cnt + rank] = v;
+    cnt += __popcll(b);
+    if (cnt > kStageCap - kWave) flush(gcnt, out);
+  }
+};
+
+__device__ __forceinline__ int wave_incl_scan_i32(int x) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane_id() >= o) x += y;
+  }
+  return x;
+}
 
 struct Range {
   int gid;  // global thread index
@@ -401,40 +466,28 @@ __device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int 
   return D.rres[e] > 0;
 }
 
-// Wave walks arcs [lo, hi) of one frontier vertex; newly labelled vertices go
-// to out[] and, when wide, to wide queue qnext.
-__device__ __forceinline__ void bfs_wave_arcs(const Dev& D, int mode, bool vB, int lo, int hi, int nb,
-                                              int32_t* out, int32_t* next_cnt, int qnext, Tally& t) {
-  const int n = D.n;
-  for (int j0 = lo; j0 < hi; j0 += kWave) {
-    const int e = j0 + lane_id();
-    bool hit = false;
-    int w = -1;
-    if (e < hi) {
-      w = D.col[e];
-      t.arc += 1;
-      if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
-        hit = atomicCAS(&D.height[w], n, nb) == n;
-        t.atom += 1;
-      }
-    }
-    int rank;
-    const int off = wave_reserve(next_cnt, hit, &rank);
-    if (hit) {
-      out[off + rank] = w;
-      const int d = degree(D, w);
-      if (d > kWideT) wq_append(D, qnext, w, d);
-    }
+// A newly labelled vertex: wide ones also enter the next level's wide queue.
+__device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int qnext) {
+  if (hit) {
+    const int d = degree(D, w);
+    if (d > kWideT) wq_append(D, qnext, w, d);
   }
 }
 
-// One BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
+// Top-down BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
 // code/dyn_pp.h:99-120; final_bfs, code/dyn_pp.h:123-140).  Frontier is
 // order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
 // get height lvl+1 (one winner per vertex through compare-and-swap) and are
-// appended after the frontier.
+// appended after the frontier through per-wave LDS staging.  The 64 frontier
+// vertices a wave holds are expanded as one flattened arc range, so
+// consecutive lanes read consecutive arcs (coalesced col/residual loads) and
+// each lane finds its arc's tail by a 6-step binary search over the wave's
+// degree prefix sums.
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
-                             int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
+                             int32_t* next_cnt, const WList& W, int qnext, Tally& t,
+                             bool plain = false) {
+  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  Stage st{sbuf[threadIdx.x / kWave], 0};
   const int n = D.n;
   const int nb = lvl + 1;
   int32_t* out = D.order + lo + cnt;
@@ -443,56 +496,150 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     int a, b;
     const int v = chunk_range(D, W, c, &a, &b);
     const bool vB = mode == BFS_PP && D.side[v] != 0;
-    bfs_wave_arcs(D, mode, vB, a, b, nb, out, next_cnt, qnext, t);
+    for (int j0 = a; j0 < b; j0 += kWave) {
+      const int e = j0 + lane_id();
+      bool hit = false;
+      int w = -1;
+      if (e < b) {
+        w = D.col[e];
+        t.arc += 1;
+        if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
+          if (plain) {
+            D.height[w] = nb;  // same value from every writer; collected by ph_bfs_collect
+          } else {
+            hit = atomicCAS(&D.height[w], n, nb) == n;
+            t.atom += 1;
+          }
+        }
+      }
+      if (!plain) {
+        bfs_note_wide(D, hit, w, qnext);
+        st.push(hit, w, next_cnt, out);
+      }
+    }
   }
-  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
+  for (int base = r.wid * kGroup; base < cnt; base += r.nw * kGroup) {
     const int i = base + lane_id();
-    int v = -1, b = 0, deg = 0;
-    bool vB = false;
-    if (i < cnt) {
-      v = D.order[lo + i];
+    int b = 0, deg = 0;
+    int vB = 0;
+    if (lane_id() < kGroup && i < cnt) {
+      const int v = D.order[lo + i];
       b = D.row[v];
       deg = D.row[v + 1] - b;
       if (deg > kWideT) deg = 0;  // handled through its chunks
-      vB = mode == BFS_PP && D.side[v] != 0;
+      vB = (mode == BFS_PP && D.side[v] != 0) ? 1 : 0;
       t.vert += 1;
     }
-    // mid-degree vertices: the whole wave walks the adjacency
-    unsigned long long big = __ballot(deg > kLaneMax);
+    const int incl = wave_incl_scan_i32(deg);
+    const int total = __shfl(incl, kWave - 1);
+    const int excl = incl - deg;
+    for (int k0 = 0; k0 < total; k0 += kWave) {
+      const int p = k0 + lane_id();
+      int pos = 0;  // number of lanes whose prefix end is <= p = owner lane
+#pragma unroll
+      for (int sh = 32; sh > 0; sh >>= 1)
+        if (__shfl(incl, pos + sh - 1) <= p) pos += sh;
+      const int ob = __shfl(b, pos & (kWave - 1));
+      const int oe = __shfl(excl, pos & (kWave - 1));
+      const int oB = __shfl(vB, pos & (kWave - 1));
+      bool hit = false;
+      int w = -1;
+      if (p < total) {
+        const int e = ob + (p - oe);
+        w = D.col[e];
+        t.arc += 1;
+        if (bfs_arc_ok(D, mode, oB != 0, e, w) && D.height[w] == n) {
+          if (plain) {
+            D.height[w] = nb;
+          } else {
+            hit = atomicCAS(&D.height[w], n, nb) == n;
+            t.atom += 1;
+          }
+        }
+      }
+      if (!plain) {
+        bfs_note_wide(D, hit, w, qnext);
+        st.push(hit, w, next_cnt, out);
+      }
+    }
+  }
+  st.flush(next_cnt, out);
+}
+
+// Frontier collection after a plain-store level: every vertex whose height is
+// now lvl+1 is appended (once, by its own lane) to order[] after the old
+// frontier, and wide ones to wide queue qnext.  One pass over height[n].
+__device__ void ph_bfs_collect(const Dev& D, const Range& r, int lvl, int32_t* next_cnt, int qnext,
+                               int32_t* out, Tally& t) {
+  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  Stage st{sbuf[threadIdx.x / kWave], 0};
+  const int nb = lvl + 1;
+  for (int base = r.wid * kWave; base < D.n; base += r.nw * kWave) {
+    const int v = base + lane_id();
+    const bool hit = v < D.n && D.height[v] == nb;
+    t.vert += 1;
+    bfs_note_wide(D, hit, v, qnext);
+    st.push(hit, v, next_cnt, out);
+  }
+  st.flush(next_cnt, out);
+}
+
+// Bottom-up BFS level over the reverse residual graph (same levels as the
+// top-down rule of staticMaxFlow_kernel_11): every unlabelled vertex w looks
+// for one arc w->x with residual > 0 into the current frontier (height ==
+// lvl) and stops at the first; no compare-and-swap is needed because w is
+// written only by its own lane.  Used for the global relabel from t when the
+// frontier is large.
+__device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* next_cnt, int qnext,
+                                int32_t* out, Tally& t) {
+  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  Stage st{sbuf[threadIdx.x / kWave], 0};
+  const int n = D.n;
+  for (int base = r.wid * kWave; base < n; base += r.nw * kWave) {
+    const int w = base + lane_id();
+    bool cand = false;
+    int b = 0, deg = 0;
+    if (w < n && D.height[w] == n) {
+      b = D.row[w];
+      deg = D.row[w + 1] - b;
+      cand = deg > 0;
+      t.vert += 1;
+    }
+    bool hit = false;
+    unsigned long long big = __ballot(cand && deg > kLaneMax);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
       const int bb = __shfl(b, leader);
       const int bd = __shfl(deg, leader);
-      const bool bB = __shfl(static_cast<int>(vB), leader) != 0;
-      bfs_wave_arcs(D, mode, bB, bb, bb + bd, nb, out, next_cnt, qnext, t);
+      bool found = false;
+      for (int j0 = 0; j0 < bd && !found; j0 += kWave) {
+        const int e = bb + j0 + lane_id();
+        bool f = false;
+        if (j0 + lane_id() < bd) {
+          t.arc += 1;
+          f = D.res[e] > 0 && D.height[D.col[e]] == lvl;
+        }
+        found = __ballot(f) != 0;
+      }
+      if (lane_id() == leader) hit = found;
     }
-    // narrow vertices: one lane each, lanes step through arcs in lockstep
-    const int my_deg = (deg <= kLaneMax) ? deg : 0;
-    int maxd = my_deg;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
-    for (int j = 0; j < maxd; ++j) {
-      bool hit = false;
-      int w = -1;
-      if (j < my_deg) {
-        const int e = b + j;
-        w = D.col[e];
+    if (cand && deg <= kLaneMax) {
+      for (int e = b; e < b + deg; ++e) {
         t.arc += 1;
-        if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
-          hit = atomicCAS(&D.height[w], n, nb) == n;
-          t.atom += 1;
+        if (D.res[e] > 0 && D.height[D.col[e]] == lvl) {
+          hit = true;
+          break;
         }
       }
-      int rank;
-      const int off = wave_reserve(next_cnt, hit, &rank);
-      if (hit) {
-        out[off + rank] = w;
-        const int d = degree(D, w);
-        if (d > kWideT) wq_append(D, qnext, w, d);
-      }
     }
+    if (hit) {
+      D.height[w] = lvl + 1;
+      if (deg > kWideT) wq_append(D, qnext, w, deg);
+    }
+    st.push(hit, w, next_cnt, out);
   }
+  st.flush(next_cnt, out);
 }
 
 // ---------------------------------------------------------------- worklists
@@ -587,81 +734,62 @@ __device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
   return true;
 }
 
-// One push-or-relabel step of staticMaxFlow_kernel_14 (code/dyn_data.h:94-129)
-// for vertex v, given the chosen arc.  Executed by one lane.
-__device__ __forceinline__ void do_push_or_relabel(const Dev& D, int v, long long ex1, int hv,
-                                                   int hh, int fe, Tally& t) {
-  if (fe < 0) return;
-  if (hv > hh) {
-    const int fec = ldr(&D.res[fe]);
-    const int d = static_cast<int>(ex1 < fec ? ex1 : fec);
-    if (d <= 0) return;
-    const int w = D.col[fe];
-    const int p = D.rev[fe];
-    add64(&D.excess[v], -static_cast<long long>(d));
-    add64(&D.excess[w], d);
-    atomicSub(&D.res[fe], d);
-    atomicAdd(&D.rres[fe], d);
-    atomicAdd(&D.res[p], d);
-    atomicSub(&D.rres[p], d);
-    t.push += 1;
-    t.atom += 6;
-  } else {
-    str(&D.height[v], hh + 1);
-    t.relabel += 1;
-  }
-}
-
-// pull_kernel step (code/dyn_pp.h:243-277): a deficit vertex pulls
-// min(-excess, residual) from its chosen neighbour, or relabels.
-__device__ __forceinline__ void do_pull_or_relabel(const Dev& D, int v, long long need, int hv,
-                                                   int hh, int fe, Tally& t) {
-  if (fe < 0) return;
-  if (hv > hh) {
-    const int rec = ldr(&D.rres[fe]);
-    const int d = static_cast<int>(need < rec ? need : rec);
-    if (d <= 0) return;
-    const int w = D.col[fe];
-    const int p = D.rev[fe];
-    add64(&D.excess[w], -static_cast<long long>(d));
-    add64(&D.excess[v], d);
-    atomicSub(&D.rres[fe], d);
-    atomicAdd(&D.res[fe], d);
-    atomicAdd(&D.rres[p], d);
-    atomicSub(&D.res[p], d);
-    t.push += 1;
-    t.atom += 6;
-  } else {
-    str(&D.height[v], hh + 1);
-    t.relabel += 1;
-  }
-}
-
-// Lane path (degree <= kLaneMax): the reference's step exactly — scan for the
-// lowest-height neighbour over residual arcs (first minimum in CSR order),
-// push to it if lower, else relabel — up to kc times.
+// Lane path (degree <= kLaneMax): up to kc discharge steps.  A step scans the
+// arcs once and pushes along every residual arc to a lower neighbour (the arcs
+// the reference's step of code/dyn_data.h:94-129 may use, one per scan there)
+// until the excess is spent; if excess remains, v relabels to (lowest residual
+// neighbour)+1, the reference's relabel rule.  Only this lane lowers excess[v]
+// and the residuals of v's arcs, so the excess read at the start bounds the
+// step's pushes and is debited with one atomic at the end.
 template <bool PULL>
 __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t) {
   const int n = D.n;
   for (int c = 0; c < kc; ++c) {
     const long long ex = ldr64(&D.excess[v]);
     const int hv = ldr(&D.height[v]);
-    const long long amt = PULL ? -ex : ex;
-    if (!(amt > 0 && (PULL || hv < n))) break;
-    int hh = INT_MAX, fe = -1;
-    for (int e = b; e < b + deg; ++e) {
+    const long long budget = PULL ? -ex : ex;
+    if (!(budget > 0 && (PULL || hv < n))) break;
+    long long moved = 0;
+    int mo = INT_MAX;
+    for (int e = b; e < b + deg && moved < budget; ++e) {
       const int w = D.col[e];
       if (!push_filter(D, pmode, w)) continue;
-      const int h1 = D.height[w];
       const int resid = PULL ? ldr(&D.rres[e]) : ldr(&D.res[e]);
-      if (h1 < hh && resid > 0) {
-        hh = h1;
-        fe = e;
+      if (resid <= 0) continue;
+      const int hw = D.height[w];
+      if (hw >= hv) {
+        mo = min(mo, hw);
+        continue;
       }
+      const long long left = budget - moved;
+      const int d = static_cast<int>(left < resid ? left : resid);
+      const int p = D.rev[e];
+      if (PULL) {
+        atomicSub(&D.rres[e], d);
+        atomicAdd(&D.res[e], d);
+        atomicAdd(&D.rres[p], d);
+        atomicSub(&D.res[p], d);
+        add64(&D.excess[w], -static_cast<long long>(d));
+      } else {
+        atomicSub(&D.res[e], d);
+        atomicAdd(&D.rres[e], d);
+        atomicAdd(&D.res[p], d);
+        atomicSub(&D.rres[p], d);
+        add64(&D.excess[w], d);
+      }
+      moved += d;
+      t.push += 1;
+      t.atom += 5;
     }
     t.arc += deg;
-    if (PULL) do_pull_or_relabel(D, v, amt, hv, hh, fe, t);
-    else do_push_or_relabel(D, v, amt, hv, hh, fe, t);
+    if (moved) {
+      add64(&D.excess[v], PULL ? moved : -moved);
+      t.atom += 1;
+    }
+    if (moved == budget) continue;  // spent; re-check for refills
+    if (mo == INT_MAX) break;       // no residual arc left
+    str(&D.height[v], mo + 1);
+    t.relabel += 1;
   }
 }
 
@@ -1207,44 +1335,57 @@ __global__ void __launch_bounds__(kBlock) k_init_residual(Dev D) { ph_init_resid
 __global__ void __launch_bounds__(kBlock) k_saturate_source(Dev D) {
   Tally t;
   ph_saturate_source(D, make_range(), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_saturate_sink_pp(Dev D) {
   Tally t;
   ph_saturate_sink_pp(D, make_range(), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_bfs_init(Dev D, int mode) {
   Tally t;
   ph_bfs_init(D, make_range(), mode, t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_bfs_init_final(Dev D, int cnt) {
   Tally t;
   ph_bfs_init_final(D, make_range(), cnt, t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, int cnt, int lvl,
-                                                      unsigned long long wpk) {
+                                                      unsigned long long wpk, int bottomup) {
   Tally t;
-  const WList W = wlist(D, lvl % 3, wpk);
-  ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
-  t.flush(D.ctrl);
+  const Range r = make_range();
+  if (bottomup == 2) {
+    const WList W = wlist(D, lvl % 3, wpk);
+    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, true);
+  } else if (bottomup) {
+    ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+  } else {
+    const WList W = wlist(D, lvl % 3, wpk);
+    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+  }
+  t.flush(D.ctrs);
+}
+__global__ void __launch_bounds__(kBlock) k_bfs_collect(Dev D, int lo, int cnt, int lvl) {
+  Tally t;
+  ph_bfs_collect(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
   Tally t;
   ph_build_wl(D, make_range(), cnt, t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_build_pp_wl(Dev D, int cnt) {
   Tally t;
   ph_build_pp_wl(D, make_range(), cnt, t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_build_final_wl(Dev D, int cnt) {
   Tally t;
   ph_build_final_wl(D, make_range(), cnt, t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 // which: 0 = all vertices (topology), 1 = wl, 2 = wl2 (pull)
 __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int kc, int pmode,
@@ -1258,7 +1399,7 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
   } else {
     ph_push<true>(D, r, D.wl2, cnt, kc, PM_ONLY_A, nullptr, wlist(D, kQPull, wpk), t);
   }
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, int rmode,
                                                    unsigned long long wpk) {
@@ -1267,18 +1408,18 @@ __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, in
   if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_static(D), t);
   else if (which == 1) ph_repair(D, r, D.wl, cnt, rmode, wlist(D, kQActive, wpk), t);
   else ph_repair(D, r, D.wl2, cnt, rmode, wlist(D, kQPull, wpk), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_pp_cut(Dev D) {
   Tally t;
   ph_pp_cut(D, make_range(), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_find_cut(Dev D) { ph_find_cut(D, make_range()); }
 __global__ void __launch_bounds__(kBlock) k_find_part(Dev D) {
   Tally t;
   ph_find_part(D, make_range(), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_find_final_cut(Dev D) { ph_find_final_cut(D, make_range()); }
 __global__ void __launch_bounds__(kBlock) k_flow(Dev D) { ph_flow(D, make_range()); }
@@ -1289,7 +1430,7 @@ __global__ void __launch_bounds__(kBlock) k_upd_apply(Dev D, int capacity_only, 
 __global__ void __launch_bounds__(kBlock) k_upd_clamp(Dev D) {
   Tally t;
   ph_upd_clamp(D, make_range(), t);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_upd_release(Dev D) { ph_upd_release(D, make_range()); }
 __global__ void __launch_bounds__(kBlock) k_cut_init(Dev D, int32_t* mark) { ph_cut_init(D, make_range(), mark); }
@@ -1330,6 +1471,19 @@ __global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
 
 __device__ __forceinline__ bool is_block0_thread0() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
+// Direction choice for one level of the relabel from t: bottom-up once the
+// frontier holds more than n / D.bottom_up vertices.
+__host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, int cnt) {
+  return mode == BFS_SINK && D.bottom_up > 0 && static_cast<long long>(cnt) * D.bottom_up > D.n;
+}
+
+// Plain-store expansion plus a collection pass over height[] once the frontier
+// holds more than n / D.collect vertices: no compare-and-swap or append atomic
+// per labelled vertex, at the price of one scan of n heights.
+__host__ __device__ __forceinline__ bool use_collect(const Dev& D, int cnt) {
+  return D.collect > 0 && static_cast<long long>(cnt) * D.collect > D.n;
+}
+
 // Global relabel with the given root mode; returns the number of vertices
 // labelled (order[0, visited)).  On entry ctrl->qcnt[] and ctrl->wq[0..2] are
 // 0; on exit they are 0 again.
@@ -1343,14 +1497,23 @@ __device__ int relabel_persistent(const Dev& D, cg::grid_group& G, const Range& 
     const int cur = lvl % 3;
     const int cnt = ldr(&D.ctrl->qcnt[cur]);
     if (cnt == 0) break;
-    const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
-    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+    if (use_bottom_up(D, mode, cnt)) {
+      ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+    } else if (use_collect(D, cnt)) {
+      const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
+      ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, true);
+      G.sync();
+      ph_bfs_collect(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+    } else {
+      const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
+      ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+    }
     if (is_block0_thread0()) {
       // slot (lvl+2)%3 was read during level lvl-1 and is next written during
       // level lvl+1
       D.ctrl->qcnt[(lvl + 2) % 3] = 0;
       D.ctrl->wq[(lvl + 2) % 3] = 0;
-      D.ctrl->ctr[C_LEVELS] += 1;
+      ctr_add(D.ctrs, C_LEVELS, 1);
     }
     lo += cnt;
     ++lvl;
@@ -1361,7 +1524,7 @@ __device__ int relabel_persistent(const Dev& D, cg::grid_group& G, const Range& 
     D.ctrl->qcnt[0] = D.ctrl->qcnt[1] = D.ctrl->qcnt[2] = 0;
     D.ctrl->wq[0] = D.ctrl->wq[1] = D.ctrl->wq[2] = 0;
     D.ctrl->visited = lo;
-    D.ctrl->ctr[C_GRELABEL] += 1;
+    ctr_add(D.ctrs, C_GRELABEL, 1);
   }
   return lo;
 }
@@ -1389,7 +1552,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
     G.sync();
     if (D.ctrl->missing || D.ctrl->invalid) {
       ph_upd_release(D, r);  // leave the state untouched, owners cleared
-      t.flush(D.ctrl);
+      t.flush(D.ctrs);
       return;  // uniform: every block read the same counters after the barrier
     }
     ph_upd_apply(D, r, false, pp);
@@ -1403,7 +1566,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
     }
     G.sync();
     ph_upd_release(D, r);
-    if (is_block0_thread0()) D.ctrl->ctr[C_UPD] += D.k;
+    if (is_block0_thread0()) ctr_add(D.ctrs, C_UPD, static_cast<unsigned long long>(D.k));
     G.sync();
   }
 
@@ -1535,13 +1698,13 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
     D.ctrl->wq[kQActive] = D.ctrl->wq[kQPull] = 0;
     D.ctrl->flag[0] = D.ctrl->flag[1] = 0;
     D.ctrl->rounds = round;
-    D.ctrl->ctr[C_ROUNDS] += round;
+    ctr_add(D.ctrs, C_ROUNDS, static_cast<unsigned long long>(round));
     D.ctrl->flow = 0;
     if (!conv) D.ctrl->status = DMF_ENOCONV;
   }
   G.sync();
   ph_flow(D, r);
-  t.flush(D.ctrl);
+  t.flush(D.ctrs);
 }
 
 }  // namespace dmfdev

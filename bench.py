@@ -206,6 +206,8 @@ def main():
     ap.add_argument("--variant", default="data")
     ap.add_argument("--mode", type=int, default=1, help="0 persistent one-launch solve, 1 stepped")
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels")
+    ap.add_argument("--sched", default="both", choices=["both", "plain"],
+                    help="time the learned scheduler beside the plain fixed schedule")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
@@ -224,19 +226,27 @@ def main():
     plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
     recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain")
     S = summarize(recs)
+    L = None
+    if args.sched != "plain":
+        learned = dmf.default_params(g, variant=args.variant, mode=1, scheduler=dmf.SCHED_LEARNED)
+        lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned")
+        L = summarize(lrecs)
+        if lflows != flows:
+            raise RuntimeError("learned scheduler produced different flows from the plain path")
     verified = None
     if not args.no_verify:
         verified = verify(g, s, t, batches, flows, which={0, len(batches) - 1})
     cpu = None if args.no_cpu else cpu_baseline(g, s, t, batches, kc, args.cpu_seconds)
 
+    head = L if L is not None else S
     out = {
         "metric": "p50 ms per update batch (RMAT-20, successive 1-10% mixed batches)",
-        "value": round(S["p50_ms"], 4),
+        "value": round(head["p50_ms"], 4),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(S["mean_ms"], 4),
+        "ms_per_step": round(head["mean_ms"], 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -249,11 +259,16 @@ def main():
             "variant": args.variant, "kc": kc, "relabel_interval": args.interval,
             "driver": "persistent" if args.mode == 0 else "stepped",
         },
-        "p99_ms": round(S["p99_ms"], 4),
-        "wall_p50_ms": round(S["wall_p50_ms"], 4),
-        "gteps": round(S["gteps"], 4),
-        "rounds_per_batch": S["rounds_mean"],
-        "launches_per_batch": S["launches_per_batch"],
+        "scheduler": "learned" if L is not None else "fixed",
+        "p99_ms": round(head["p99_ms"], 4),
+        "wall_p50_ms": round(head["wall_p50_ms"], 4),
+        "gteps": round(head["gteps"], 4),
+        "rounds_per_batch": head["rounds_mean"],
+        "plain": {"p50_ms": round(S["p50_ms"], 4), "p99_ms": round(S["p99_ms"], 4),
+                  "gteps": round(S["gteps"], 4), "rounds_per_batch": S["rounds_mean"],
+                  "global_relabels_per_batch": S["global_relabels_mean"],
+                  "roofline_frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4)},
+        "learned_speedup_p50": round(S["p50_ms"] / L["p50_ms"], 3) if L is not None else None,
         "flows_verified_vs_dinic": verified,
         "roofline": {
             "bound": "hbm",

@@ -35,8 +35,13 @@
 
 #include "../../include/dmf.h"
 #include "phases.hpp"
+#include "sched.hpp"
 
 using namespace dmfdev;
+
+namespace {
+#include "sched_weights.inc"
+}
 
 // ============================================================ host engine
 
@@ -58,6 +63,15 @@ struct dmf_engine {
   dmf_counters last{};
   int64_t bytes = 0;
   uint64_t launches = 0;
+  // learned scheduler
+  SchedWeights* d_sched_w = nullptr;
+  SchedState* d_sched = nullptr;
+  SchedState* h_sched = nullptr;  // pinned mirror
+  int last_levels = 0;
+  int last_variant = -1;          // variant of the last solve (-1 = none yet)
+  FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
+  uint64_t solve_id = 0;
+  std::chrono::steady_clock::time_point t_solve;
 };
 
 namespace {
@@ -152,6 +166,7 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
   }
   E->last.bfs_levels += lvl;
   E->last.global_relabels += 1;
+  E->last_levels = lvl;
   *visited = lo;
   return DMF_OK;
 }
@@ -164,18 +179,48 @@ int reset_lists(dmf_engine* E) {
   return DMF_OK;
 }
 
+double since_solve_ms(const dmf_engine* E) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - E->t_solve).count();
+}
+
+// Evaluates the scheduler MLP on the device and reads its decision back.
+int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int round, int variant,
+              int relabelled) {
+  Dev& D = E->D;
+  const int k = D.k;
+  const float inc = k > 0 ? static_cast<float>(E->h_ctrl->inc) / static_cast<float>(k) : 0.5f;
+  hipLaunchKernelGGL(k_sched, dim3(1), dim3(kWave), 0, E->stream, D, E->d_sched_w, E->d_sched, phase, k, inc,
+                     rounds_since, E->last_levels);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(E->h_sched, E->d_sched, sizeof(SchedState), hipMemcpyDeviceToHost, E->stream));
+  HIP_OK(hipStreamSynchronize(E->stream));
+  if (E->trace) {
+    const SchedState& S = *E->h_sched;
+    std::fprintf(E->trace,
+                 "{\"solve\":%llu,\"what\":\"%s\",\"round\":%d,\"variant\":%d,\"relabelled\":%d,"
+                 "\"k\":%d,\"t_ms\":%.4f,\"feat\":[%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g],"
+                 "\"out\":[%.6g,%.6g,%.6g,%.6g]}\n",
+                 (unsigned long long)E->solve_id, what, round, variant, relabelled, k, since_solve_ms(E),
+                 S.feat[0], S.feat[1], S.feat[2], S.feat[3], S.feat[4], S.feat[5], S.feat[6], S.feat[7],
+                 S.out[0], S.out[1], S.out[2], S.out[3]);
+  }
+  return DMF_OK;
+}
+
 // Stepped solve (one kernel per phase, host-checked conditions).
-int solve_stepped(dmf_engine* E, const Cfg& C) {
+int solve_stepped(dmf_engine* E, const Cfg& C0) {
   Dev& D = E->D;
   int rc;
-  const bool pp = C.variant == DMF_VARIANT_PP;
-  const bool topo = C.variant == DMF_VARIANT_TOPO;
+  Cfg C = C0;
+  const bool learned = C.scheduler == DMF_SCHED_LEARNED;
+  const bool consult = learned || E->trace != nullptr;
   const int n = E->n;
   const int64_t static_chunks = chunks_of(D.wide_packed);
   if (C.entry == 0) {
     LAUNCH(k_init_residual, blocks_for(std::max(E->m, E->n)), D);
     LAUNCH(k_saturate_source, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
-    if (pp) LAUNCH(k_saturate_sink_pp, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
+    if (C.variant == DMF_VARIANT_PP) LAUNCH(k_saturate_sink_pp, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
   } else if (C.entry == 1) {
     LAUNCH(k_upd_lookup, blocks_for(D.k), D);
     if ((rc = sync_ctrl(E))) return rc;
@@ -183,6 +228,16 @@ int solve_stepped(dmf_engine* E, const Cfg& C) {
       LAUNCH(k_upd_release, blocks_for(D.k), D);
       return DMF_OK;
     }
+    if (consult) {
+      // batch start: the scheduler picks the variant (push-pull needs the cut
+      // of a previous solve, which every solve leaves in side[])
+      if ((rc = run_sched(E, 0, 0, "batch", 0, C.variant, 0))) return rc;
+      if (learned && E->h_sched->valid && E->last_variant >= 0) C.variant = E->h_sched->variant;
+    }
+  }
+  const bool pp = C.variant == DMF_VARIANT_PP;
+  const bool topo = C.variant == DMF_VARIANT_TOPO;
+  if (C.entry == 1) {
     LAUNCH(k_upd_apply, blocks_for(std::max<int64_t>(D.k, D.wide_packed >> 32)), D, 0, pp ? 1 : 0);
     if (pp) {
       LAUNCH(k_pp_cut, phase_blocks(n, static_chunks), D);
@@ -197,9 +252,11 @@ int solve_stepped(dmf_engine* E, const Cfg& C) {
   if ((rc = reset_lists(E))) return rc;
   if (!pp || C.entry == 0) {
     int since = C.interval, visited = 0;
+    bool want_relabel = true;  // scheduler decision for the coming round
     for (;;) {
       bool fresh = false;
-      if (since >= C.interval) {
+      const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
+      if (due) {
         if ((rc = relabel_stepped(E, BFS_SINK, 0, &visited))) return rc;
         since = 0;
         fresh = true;
@@ -234,8 +291,13 @@ int solve_stepped(dmf_engine* E, const Cfg& C) {
       }
       ++round;
       ++since;
+      if (consult) {
+        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0))) return rc;
+        want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
+      }
     }
-    if (pp && conv) LAUNCH(k_find_cut, blocks_for(n), D);
+    // every solve leaves a cut in side[] so that a push-pull batch may follow
+    if (conv) LAUNCH(k_find_cut, blocks_for(n), D);
   } else {
     for (;;) {
       int visited = 0;
@@ -289,6 +351,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C) {
   if ((rc = zero_dev(E, &D.ctrl->flow, sizeof(long long)))) return rc;
   LAUNCH(k_flow, blocks_for(n), D);
   E->last.rounds = round;
+  E->last_variant = C.variant;
   if (!conv) return DMF_ENOCONV;
   return DMF_OK;
 }
@@ -296,6 +359,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C) {
 int solve_persistent(dmf_engine* E, const Cfg& C) {
   Dev D = E->D;
   Cfg c = C;
+  E->last_variant = C.variant;
   void* args[] = {&D, &c};
   hipError_t err = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_solve_persistent),
                                               dim3(E->coop_blocks), dim3(kBlock), args, 0,
@@ -325,6 +389,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
   c.interval = p->relabel_interval > 0 ? p->relabel_interval : 1;
   c.max_rounds = p->max_rounds > 0 ? p->max_rounds : 4 * E->n + 1000;
   c.entry = entry;
+  c.scheduler = p->scheduler;
   return c;
 }
 
@@ -376,16 +441,31 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
 
 int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   const Cfg C = make_cfg(E, p, entry);
-  if (C.variant < 0 || C.variant > 2) return DMF_EINVAL;
+  if (C.variant < 0 || C.variant > 2 || C.scheduler < 0 || C.scheduler > 1) return DMF_EINVAL;
   int rc = begin_solve(E);
   if (rc) return rc;
   HIP_OK(hipEventRecord(E->ev0, E->stream));
-  const bool stepped = (p && p->mode == 1) || E->coop_blocks <= 0;
+  ++E->solve_id;
+  E->t_solve = std::chrono::steady_clock::now();
+  // the scheduler runs between rounds on the host-driven path
+  const bool stepped = (p && p->mode == 1) || E->coop_blocks <= 0 || C.scheduler == DMF_SCHED_LEARNED ||
+                       E->trace != nullptr;
   int rc_driver = stepped ? solve_stepped(E, C) : solve_persistent(E, C);
   if (rc_driver == DMF_EDEVICE) return rc_driver;
   HIP_OK(hipEventRecord(E->ev1, E->stream));
   if (entry == 1) E->last.updates = static_cast<uint64_t>(E->D.k);
-  return finish_solve(E, rc_driver, flow);
+  rc = finish_solve(E, rc_driver, flow);
+  if (E->trace) {
+    std::fprintf(E->trace,
+                 "{\"solve\":%llu,\"what\":\"summary\",\"entry\":%d,\"variant\":%d,\"scheduler\":%d,"
+                 "\"interval\":%d,\"k\":%d,\"device_ms\":%.4f,\"wall_ms\":%.4f,\"rounds\":%llu,"
+                 "\"relabels\":%llu,\"rc\":%d}\n",
+                 (unsigned long long)E->solve_id, entry, E->last_variant, C.scheduler, C.interval, E->D.k,
+                 E->last.device_ms, since_solve_ms(E), (unsigned long long)E->last.rounds,
+                 (unsigned long long)E->last.global_relabels, rc);
+    std::fflush(E->trace);
+  }
+  return rc;
 }
 
 int ensure_update_capacity(dmf_engine* E, int32_t k) {
@@ -461,6 +541,9 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
   int32_t *row, *col, *rev, *cap, *res, *rres, *height, *order, *wl, *wl2, *part, *owner, *hval;
   int32_t *wq_vert, *wq_base, *wide_vert, *wide_base;
   unsigned long long* ctrs;
+  uint32_t* widebits;
+  std::vector<uint32_t> wbits((n + 31) / 32 + 1, 0u);
+  for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
   long long* excess;
   uint8_t *side, *pfinal;
   Ctrl* ctrl;
@@ -474,7 +557,8 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
       (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &hkey, hcap)) || (rc = dalloc(E, &hval, hcap)) ||
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &wq_vert, nwide * kNumQ)) ||
       (rc = dalloc(E, &wq_base, nwide * kNumQ)) || (rc = dalloc(E, &wide_vert, nwide)) ||
-      (rc = dalloc(E, &wide_base, nwide)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots))) {
+      (rc = dalloc(E, &wide_base, nwide)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
+      (rc = dalloc(E, &widebits, wbits.size()))) {
     dmf_engine_destroy(E);
     return rc;
   }
@@ -484,6 +568,7 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
   D.hmask = hcap - 1; D.owner = owner;
   D.wq_vert = wq_vert; D.wq_base = wq_base; D.wq_cap = static_cast<int32_t>(nwide);
   D.ctrs = ctrs;
+  D.widebits = widebits;
   {
     const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
     D.bottom_up = bu ? std::atoi(bu) : 0;
@@ -518,6 +603,8 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
             hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
             hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess &&
             hipMemsetAsync(hval, 0xff, sizeof(int32_t) * hcap, E->stream) == hipSuccess &&
+            hipMemcpyAsync(widebits, wbits.data(), sizeof(uint32_t) * wbits.size(), hipMemcpyHostToDevice,
+                           E->stream) == hipSuccess &&
             (wide_v.empty() ||
              (hipMemcpyAsync(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size(), hipMemcpyHostToDevice,
                              E->stream) == hipSuccess &&
@@ -545,6 +632,17 @@ int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) 
     const int want = cap ? std::max(1, std::atoi(cap)) : 8;
     E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
   }
+  // scheduler weights (compiled in) and state
+  if (hipMalloc(reinterpret_cast<void**>(&E->d_sched_w), sizeof(SchedWeights)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&E->d_sched), sizeof(SchedState)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&E->h_sched), sizeof(SchedState), hipHostMallocDefault) != hipSuccess ||
+      hipMemcpy(E->d_sched_w, kSchedWeightsInit, sizeof(SchedWeights), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(E->d_sched, 0, sizeof(SchedState)) != hipSuccess) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  E->bytes += sizeof(SchedWeights) + sizeof(SchedState);
+  if (const char* tp = std::getenv("DMF_TRACE")) E->trace = std::fopen(tp, "a");
   *out = E;
   return DMF_OK;
 }
@@ -573,6 +671,7 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(D.hval);
   hipFree(D.owner);
   hipFree(D.ctrs);
+  hipFree(const_cast<uint32_t*>(D.widebits));
   hipFree(D.wq_vert);
   hipFree(D.wq_base);
   hipFree(const_cast<int32_t*>(D.wide_vert));
@@ -582,6 +681,10 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(E->d_uw);
   hipFree(E->d_uarc);
   if (E->h_ctrl) hipHostFree(E->h_ctrl);
+  if (E->h_sched) hipHostFree(E->h_sched);
+  hipFree(E->d_sched_w);
+  hipFree(E->d_sched);
+  if (E->trace) std::fclose(E->trace);
   if (E->ev0) hipEventDestroy(E->ev0);
   if (E->ev1) hipEventDestroy(E->ev1);
   if (E->own_stream && E->stream) hipStreamDestroy(E->stream);

@@ -72,7 +72,8 @@ struct Ctrl {
   int32_t invalid;    // updates with out-of-range ids
   int32_t visited;    // vertices labelled by the last global relabel
   int32_t rounds;     // rounds of the last solve
-  int32_t pad[3];
+  int32_t inc;        // updates that raise an arc's capacity (scheduler feature)
+  int32_t pad[2];
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
   long long flow;
 };
@@ -108,6 +109,7 @@ struct Dev {
   const int32_t* wide_vert;
   const int32_t* wide_base;
   unsigned long long wide_packed;
+  const uint32_t* widebits;  // bit v set when degree(v) > kWideT
   // current update batch
   const int32_t* usrc;
   const int32_t* udst;
@@ -123,6 +125,7 @@ struct Cfg {
   int32_t interval;
   int32_t max_rounds;
   int32_t entry;  // 0 = static (init + saturate), 1 = batch (apply + repair)
+  int32_t scheduler;  // DMF_SCHED_*
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;
@@ -311,14 +314,30 @@ __device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
   D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu);
 }
 
-// Chunk c of a wide list -> vertex and arc range [lo, hi).
-__device__ __forceinline__ int chunk_range(const Dev& D, const WList& L, int c, int* lo, int* hi) {
-  int a = 0, b = L.slots - 1;
-  while (a < b) {
-    const int mid = (a + b + 1) >> 1;
-    if (L.base[mid] <= c) a = mid;
-    else b = mid - 1;
+// Largest slot a with base[a] <= c, found by the whole wave: each step every
+// lane tests one of 64 evenly spaced candidates, so a list of up to 4096 slots
+// takes two dependent loads instead of a 12-step scalar binary search.
+// Must be called by all 64 lanes with the same c.
+__device__ __forceinline__ int wave_find_slot(const int32_t* base, int slots, int c) {
+  int lo = 0, span = slots;  // invariant: base[lo] <= c, answer in [lo, lo+span)
+  while (span > kWave) {
+    const int step = (span + kWave - 1) / kWave;
+    const int j = lo + lane_id() * step;
+    const bool le = lane_id() * step < span && base[j] <= c;
+    const unsigned long long b = __ballot(le);
+    const int last = 63 - __clzll(static_cast<long long>(b));
+    const int end = lo + span;
+    lo += last * step;
+    span = min(step, end - lo);
   }
+  const bool le = lane_id() < span && base[lo + lane_id()] <= c;
+  const unsigned long long b = __ballot(le);
+  return lo + (63 - __clzll(static_cast<long long>(b)));
+}
+
+// Chunk c of a wide list -> vertex and arc range [lo, hi).  All 64 lanes.
+__device__ __forceinline__ int chunk_range(const Dev& D, const WList& L, int c, int* lo, int* hi) {
+  const int a = wave_find_slot(L.base, L.slots, c);
   const int v = L.vert[a];
   const int beg = D.row[v];
   *lo = beg + (c - L.base[a]) * kChunk;
@@ -467,11 +486,11 @@ __device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int 
 }
 
 // A newly labelled vertex: wide ones also enter the next level's wide queue.
+__device__ __forceinline__ bool is_wide(const Dev& D, int v) {
+  return (D.widebits[static_cast<unsigned>(v) >> 5] >> (v & 31)) & 1u;
+}
 __device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int qnext) {
-  if (hit) {
-    const int d = degree(D, w);
-    if (d > kWideT) wq_append(D, qnext, w, d);
-  }
+  if (hit && is_wide(D, w)) wq_append(D, qnext, w, degree(D, w));
 }
 
 // Top-down BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
@@ -533,33 +552,46 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     const int incl = wave_incl_scan_i32(deg);
     const int total = __shfl(incl, kWave - 1);
     const int excl = incl - deg;
-    for (int k0 = 0; k0 < total; k0 += kWave) {
-      const int p = k0 + lane_id();
-      int pos = 0;  // number of lanes whose prefix end is <= p = owner lane
+    // two 64-arc tiles per iteration keep twice the loads in flight
+    for (int k0 = 0; k0 < total; k0 += 2 * kWave) {
+      const int pA = k0 + lane_id(), pB = pA + kWave;
+      int posA = 0, posB = 0;  // owner lanes: number of group lanes whose prefix end <= p
 #pragma unroll
-      for (int sh = 32; sh > 0; sh >>= 1)
-        if (__shfl(incl, pos + sh - 1) <= p) pos += sh;
-      const int ob = __shfl(b, pos & (kWave - 1));
-      const int oe = __shfl(excl, pos & (kWave - 1));
-      const int oB = __shfl(vB, pos & (kWave - 1));
-      bool hit = false;
-      int w = -1;
-      if (p < total) {
-        const int e = ob + (p - oe);
-        w = D.col[e];
-        t.arc += 1;
-        if (bfs_arc_ok(D, mode, oB != 0, e, w) && D.height[w] == n) {
-          if (plain) {
-            D.height[w] = nb;
-          } else {
-            hit = atomicCAS(&D.height[w], n, nb) == n;
-            t.atom += 1;
-          }
-        }
+      for (int sh = kGroup / 2; sh > 0; sh >>= 1) {
+        if (__shfl(incl, posA + sh - 1) <= pA) posA += sh;
+        if (__shfl(incl, posB + sh - 1) <= pB) posB += sh;
       }
-      if (!plain) {
-        bfs_note_wide(D, hit, w, qnext);
-        st.push(hit, w, next_cnt, out);
+      posA &= kGroup - 1;
+      posB &= kGroup - 1;
+      const int eA = __shfl(b, posA) + (pA - __shfl(excl, posA));
+      const int eB = __shfl(b, posB) + (pB - __shfl(excl, posB));
+      const bool okA = __shfl(vB, posA) != 0, okB = __shfl(vB, posB) != 0;
+      const bool inA = pA < total, inB = pB < total;
+      int wA = -1, wB = -1;
+      bool cA = false, cB = false;
+      if (inA) {
+        wA = D.col[eA];
+        cA = bfs_arc_ok(D, mode, okA, eA, wA);
+      }
+      if (inB) {
+        wB = D.col[eB];
+        cB = bfs_arc_ok(D, mode, okB, eB, wB);
+      }
+      cA = cA && D.height[wA] == n;
+      cB = cB && D.height[wB] == n;
+      t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
+      bool hitA = false, hitB = false;
+      if (plain) {
+        if (cA) D.height[wA] = nb;  // same value from every writer; collected by ph_bfs_collect
+        if (cB) D.height[wB] = nb;
+      } else {
+        if (cA) hitA = atomicCAS(&D.height[wA], n, nb) == n;
+        if (cB) hitB = atomicCAS(&D.height[wB], n, nb) == n;
+        t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
+        bfs_note_wide(D, hitA, wA, qnext);
+        bfs_note_wide(D, hitB, wB, qnext);
+        st.push(hitA, wA, next_cnt, out);
+        st.push(hitB, wB, next_cnt, out);
       }
     }
   }
@@ -1191,7 +1223,7 @@ __device__ int lookup_arc(const Dev& D, int u, int v) {
 // OnAdd_kernel arc search (code/dyn_data.h:191-201) by hash lookup, plus
 // last-record-wins resolution of duplicate arcs within the batch.
 __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
-  int missing = 0, invalid = 0;
+  int missing = 0, invalid = 0, inc = 0;
   for (int i = r.gid; i < D.k; i += r.gsz) {
     const int a = D.usrc[i], b = D.udst[i];
     int arc = -2;
@@ -1199,13 +1231,18 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
       ++invalid;
     } else if (a != b) {  // self-loops skipped (code/update.hpp:82, code/dyn_data.h:194)
       arc = lookup_arc(D, a, b);
-      if (arc < 0) ++missing;
-      else atomicMax(&D.owner[arc], i);
+      if (arc < 0) {
+        ++missing;
+      } else {
+        atomicMax(&D.owner[arc], i);
+        inc += D.uw[i] > D.cap[arc];
+      }
     }
     D.uarc[i] = arc;
   }
   if (missing) atomicAdd(&D.ctrl->missing, missing);
   if (invalid) atomicAdd(&D.ctrl->invalid, invalid);
+  if (inc) atomicAdd(&D.ctrl->inc, inc);
 }
 
 // OnAdd_kernel update (code/dyn_data.h:202-205) for the winning record of each

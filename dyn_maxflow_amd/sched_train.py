@@ -1,0 +1,229 @@
+"""Trains the learned scheduler from traces the engine itself generates.
+
+  python -m dyn_maxflow_amd.sched_train collect --out DIR   (needs the GPU)
+  python -m dyn_maxflow_amd.sched_train train --traces DIR  (CPU; writes csrc/sched_weights.inc)
+
+collect: for a set of training workloads (R-MAT scale 15-18 and small grids,
+several batch sizes and increase/decrease mixes, none of them the RMAT-20
+bench graph) every batch is solved under every fixed schedule in
+{data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}.  The engine writes one
+trace line per scheduler evaluation (the 8 device-computed features, the round,
+whether that round started with a global relabel) and one summary line per
+solve (device ms, variant, interval).
+
+train: for every (workload, batch) the fastest schedule is the teacher.  The
+variant head learns that schedule's variant from the batch-start features; the
+relabel head learns, from the features at the end of each round of the
+teacher's run, whether the teacher relabelled before the next round.  A
+numpy MLP 8->32->32->4 (ReLU) is fitted with Adam; the weights are written in
+the row-major SchedWeights order of csrc/sched.hpp.
+"""
+from __future__ import annotations
+
+import argparse
+import glob
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+WEIGHTS = os.path.join(HERE, "csrc", "sched_weights.inc")
+FEAT, HID, OUT = 8, 32, 4
+VARIANTS = ["data", "topo", "pp"]
+INTERVALS = [1, 2, 3, 5, 8]
+
+
+def _workloads():
+    import dyn_maxflow_amd as dmf
+    out = []
+    for scale, seed in ((15, 11), (16, 12), (17, 13), (18, 14)):
+        u, v, w = dmf.gen_rmat(scale, 16, seed)
+        out.append((f"rmat{scale}s{seed}", u, v, w, 1, 100))
+    for side, seed in ((128, 21), (256, 22)):
+        u, v, w = dmf.gen_grid(side, side, seed, 1, 1000)
+        out.append((f"grid{side}s{seed}", u, v, w, 1, 1000))
+    return out
+
+
+def collect(out_dir: str, quick: bool = False) -> None:
+    sys.path.insert(0, ROOT)
+    import dyn_maxflow_amd as dmf
+    os.makedirs(out_dir, exist_ok=True)
+    for name, u, v, w, lo, hi in _workloads():
+        g = dmf.build_csr(u, v, w)
+        if name.startswith("grid"):
+            s, t = 0, g.n - 1
+        else:
+            s, t = dmf.pick_source_sink(g.n, u, v)
+        batches = []
+        cur = w.copy()
+        for i, (pct, inc) in enumerate(((1, 0.5), (3, 0.5), (5, 0.2), (10, 0.5), (2, 0.8), (7, 0.5))):
+            src, dst, nw = dmf.gen_updates(u, v, cur, max(1, len(u) * pct // 100), 7000 + i,
+                                           inc_frac=inc, cap_lo=lo, cap_hi=hi)
+            batches.append((src, dst, nw))
+        trace = os.path.join(out_dir, f"{name}.jsonl")
+        if os.path.exists(trace):
+            os.remove(trace)
+        os.environ["DMF_TRACE"] = trace
+        with dmf.Engine(g, s, t) as eng:
+            os.environ.pop("DMF_TRACE", None)
+            base = dmf.default_params(g)
+            for variant in VARIANTS:
+                for interval in ([1, 3] if quick else INTERVALS):
+                    if variant == "pp" and interval > 1:
+                        continue  # the push-pull batch loop relabels every round
+                    eng.reset()
+                    eng.static_solve(base)
+                    p = dmf.default_params(g, variant=variant, relabel_interval=interval)
+                    for src, dst, nw in batches:
+                        eng.apply_batch(src, dst, nw, p)
+                    print(f"[collect] {name} {variant} interval={interval}", flush=True)
+
+
+def _load(trace_dir: str):
+    """Returns per-file lists of solves: dict(summary, rows)."""
+    files = sorted(glob.glob(os.path.join(trace_dir, "*.jsonl")))
+    data = {}
+    for f in files:
+        solves = {}
+        for line in open(f):
+            rec = json.loads(line)
+            solves.setdefault(rec["solve"], {"rows": []})
+            if rec["what"] == "summary":
+                solves[rec["solve"]]["summary"] = rec
+            else:
+                solves[rec["solve"]]["rows"].append(rec)
+        data[os.path.basename(f)] = [solves[k] for k in sorted(solves)]
+    return data
+
+
+def _examples(data):
+    """(X_relabel, y_relabel, X_variant, y_variant) from the teacher schedules."""
+    Xr, yr, Xv, yv = [], [], [], []
+    for fname, solves in data.items():
+        # batch solves grouped by schedule, in order; static solves (entry 0) skipped
+        runs = {}
+        for sv in solves:
+            sm = sv.get("summary")
+            if not sm or sm["entry"] != 1 or sm["scheduler"] != 0:
+                continue
+            key = (sm["variant"], sm["interval"])
+            runs.setdefault(key, []).append(sv)
+        if not runs:
+            continue
+        nb = min(len(r) for r in runs.values())
+        for b in range(nb):
+            best = min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
+            teacher = runs[best][b]
+            batch_rows = [r for r in teacher["rows"] if r["what"] == "batch"]
+            if batch_rows:
+                Xv.append(batch_rows[0]["feat"])
+                yv.append(best[0])
+            rows = [r for r in teacher["rows"] if r["what"] == "round"]
+            for i, r in enumerate(rows):
+                nxt = rows[i + 1]["relabelled"] if i + 1 < len(rows) else 1
+                Xr.append(r["feat"])
+                yr.append(nxt)
+    return (np.array(Xr, np.float32), np.array(yr, np.float32), np.array(Xv, np.float32),
+            np.array(yv, np.int64))
+
+
+def _forward(P, X):
+    h1 = np.maximum(0, X @ P["w1"] + P["b1"])
+    h2 = np.maximum(0, h1 @ P["w2"] + P["b2"])
+    return h1, h2, h2 @ P["w3"] + P["b3"]
+
+
+def fit(Xr, yr, Xv, yv, steps=4000, lr=3e-3, seed=0):
+    rng = np.random.default_rng(seed)
+    P = {
+        "w1": rng.normal(0, np.sqrt(2 / FEAT), (FEAT, HID)).astype(np.float32),
+        "b1": np.zeros(HID, np.float32),
+        "w2": rng.normal(0, np.sqrt(2 / HID), (HID, HID)).astype(np.float32),
+        "b2": np.zeros(HID, np.float32),
+        "w3": rng.normal(0, np.sqrt(1 / HID), (HID, OUT)).astype(np.float32),
+        "b3": np.zeros(OUT, np.float32),
+    }
+    m = {k: np.zeros_like(v) for k, v in P.items()}
+    s2 = {k: np.zeros_like(v) for k, v in P.items()}
+    X = np.concatenate([Xr, Xv]) if len(Xv) else Xr
+    nr = len(Xr)
+    for it in range(1, steps + 1):
+        h1, h2, o = _forward(P, X)
+        g = np.zeros_like(o)
+        # relabel head: logistic loss on rows [0, nr)
+        if nr:
+            pr = 1 / (1 + np.exp(-np.clip(o[:nr, 0], -30, 30)))
+            g[:nr, 0] = (pr - yr) / nr
+        # variant head: softmax over out[1:4] on rows [nr, ...)
+        if len(Xv):
+            z = o[nr:, 1:4]
+            z = z - z.max(axis=1, keepdims=True)
+            e = np.exp(z)
+            sm = e / e.sum(axis=1, keepdims=True)
+            sm[np.arange(len(yv)), yv] -= 1
+            g[nr:, 1:4] = sm / len(yv)
+        grads = {"w3": h2.T @ g, "b3": g.sum(0)}
+        d2 = (g @ P["w3"].T) * (h2 > 0)
+        grads["w2"] = h1.T @ d2
+        grads["b2"] = d2.sum(0)
+        d1 = (d2 @ P["w2"].T) * (h1 > 0)
+        grads["w1"] = X.T @ d1
+        grads["b1"] = d1.sum(0)
+        for k in P:
+            m[k] = 0.9 * m[k] + 0.1 * grads[k]
+            s2[k] = 0.999 * s2[k] + 0.001 * grads[k] ** 2
+            mh = m[k] / (1 - 0.9 ** it)
+            sh = s2[k] / (1 - 0.999 ** it)
+            P[k] = (P[k] - lr * mh / (np.sqrt(sh) + 1e-8)).astype(np.float32)
+    _, _, o = _forward(P, X)
+    acc_r = float(((o[:nr, 0] > 0) == (yr > 0.5)).mean()) if nr else float("nan")
+    acc_v = float((o[nr:, 1:4].argmax(1) == yv).mean()) if len(Xv) else float("nan")
+    return P, acc_r, acc_v
+
+
+def write_weights(P, note: str) -> None:
+    flat = np.concatenate([P["w1"].ravel(), P["b1"], P["w2"].ravel(), P["b2"], P["w3"].ravel(), P["b3"]])
+    lines = ["// Scheduler weights for sched.hpp, row-major in SchedWeights order.",
+             f"// Trained by dyn_maxflow_amd/sched_train.py: {note}",
+             "#define DMF_SCHED_TRAINED 1",
+             f"static const float kSchedWeightsInit[{len(flat)}] = {{"]
+    for i in range(0, len(flat), 6):
+        lines.append("  " + ", ".join(f"{float(x)!r}f" for x in flat[i:i + 6]) + ",")
+    lines.append("};")
+    with open(WEIGHTS, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def train(trace_dir: str) -> None:
+    data = _load(trace_dir)
+    Xr, yr, Xv, yv = _examples(data)
+    print(f"[train] relabel examples {len(Xr)} (positive {yr.mean() if len(yr) else 0:.2f}), "
+          f"variant examples {len(Xv)} (counts {np.bincount(yv, minlength=3).tolist()})")
+    P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
+    print(f"[train] accuracy: relabel {acc_r:.3f} variant {acc_v:.3f}")
+    write_weights(P, f"{len(Xr)} relabel / {len(Xv)} variant examples, train acc "
+                     f"{acc_r:.3f} / {acc_v:.3f}")
+    print(f"[train] wrote {WEIGHTS}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--out", required=True)
+    c.add_argument("--quick", action="store_true")
+    t = sub.add_parser("train")
+    t.add_argument("--traces", required=True)
+    a = ap.parse_args()
+    if a.cmd == "collect":
+        collect(a.out, a.quick)
+    else:
+        train(a.traces)
+
+
+if __name__ == "__main__":
+    main()

@@ -201,3 +201,33 @@ def test_counters_are_populated(gpu):
         c = eng.counters()
         assert c["updates"] == len(src)
         assert eng.device_bytes() > 20 * g.m
+
+
+def test_learned_scheduler_keeps_results(gpu, tmp_path):
+    """Whatever the scheduler decides (variant per batch, relabel timing), the
+    flow value and S_min must stay exact; its trace must be written."""
+    dmf = gpu
+    import json
+    g, s, t, (u, v, w) = rmat_case(12, 16, 9)
+    trace = tmp_path / "trace.jsonl"
+    os.environ["DMF_TRACE"] = str(trace)
+    try:
+        eng = dmf.Engine(g, s, t)
+    finally:
+        os.environ.pop("DMF_TRACE", None)
+    with eng:
+        p = dmf.default_params(g, scheduler=dmf.SCHED_LEARNED)
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b in range(4):
+            src, dst, nw = dmf.gen_updates(u, v, cur, len(u) * (b + 1) // 50, 900 + b)
+            applied.append((src, dst, nw))
+            say(f"[learned] batch {b}")
+            f = eng.apply_batch(src, dst, nw, p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"learned batch {b}")
+    recs = [json.loads(l) for l in open(trace)]
+    assert any(r["what"] == "round" for r in recs)
+    assert sum(r["what"] == "summary" for r in recs) == 5
+    assert all(len(r["feat"]) == 8 for r in recs if r["what"] != "summary")

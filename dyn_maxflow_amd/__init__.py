@@ -148,6 +148,12 @@ def _lib():
                                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I64P,
                                              C.POINTER(C.c_double)]),
         "dmf_device_count": (C.c_int, []),
+        "dmf_engine_autotune": (C.c_int, [C.c_void_p, C.POINTER(Params), _I32P, _I32P]),
+        "dmf_batch_create": (C.c_int, [C.c_int32, C.POINTER(_Csr), _I32P, _I32P, C.POINTER(C.c_void_p)]),
+        "dmf_engine_instances": (C.c_int32, [C.c_void_p]),
+        "dmf_batch_static_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), _I64P]),
+        "dmf_batch_apply": (C.c_int, [C.c_void_p, _I32P, _I32P, _I32P, _I32P, C.POINTER(Params), _I64P]),
+        "dmf_batch_flows": (C.c_int, [C.c_void_p, _I64P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -393,6 +399,61 @@ class Engine:
 
     def set_stream(self, stream_ptr: int) -> None:
         _check(_lib().dmf_engine_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def autotune(self, params: Params):
+        """Picks the fastest (lane_max, wide_t) degree-bucket thresholds; resets the engine."""
+        a, b = C.c_int32(), C.c_int32()
+        _check(_lib().dmf_engine_autotune(self._h, C.byref(params), C.byref(a), C.byref(b)), "autotune")
+        return a.value, b.value
+
+
+class BatchEngine(Engine):
+    """Many independent (graph, s, t) instances resident as one disjoint union
+    (dmf_batch_create); every solve covers all of them at once."""
+
+    def __init__(self, graphs, sources, sinks):
+        self.graphs = list(graphs)
+        k = len(self.graphs)
+        self._csrs = (_Csr * k)(*[g._csr() for g in self.graphs])
+        s = _i32(sources)
+        t = _i32(sinks)
+        self._h = C.c_void_p()
+        _check(_lib().dmf_batch_create(k, self._csrs, _ptr(s), _ptr(t), C.byref(self._h)), "batch_create")
+        self.k = k
+        self.offsets = np.concatenate([[0], np.cumsum([g.n for g in self.graphs])]).astype(np.int64)
+        n = int(self.offsets[-1])
+        self.g = Graph(n, sum(g.m for g in self.graphs), np.zeros(n + 1, np.int32), np.zeros(0, np.int32),
+                       np.zeros(0, np.int32), np.zeros(0, np.int32))
+
+    def static_solve_all(self, params: Params) -> np.ndarray:
+        f = np.zeros(self.k, np.int64)
+        _check(_lib().dmf_batch_static_solve(self._h, C.byref(params), f.ctypes.data_as(_I64P)),
+               "batch_static_solve")
+        return f
+
+    def apply_all(self, updates, params: Params) -> np.ndarray:
+        """updates: one (src, dst, w) per instance, instance-local ids."""
+        counts = _i32([len(u[0]) for u in updates])
+        src = _i32(np.concatenate([u[0] for u in updates])) if len(updates) else _i32([])
+        dst = _i32(np.concatenate([u[1] for u in updates])) if len(updates) else _i32([])
+        w = _i32(np.concatenate([u[2] for u in updates])) if len(updates) else _i32([])
+        f = np.zeros(self.k, np.int64)
+        _check(_lib().dmf_batch_apply(self._h, _ptr(counts), _ptr(src), _ptr(dst), _ptr(w),
+                                      C.byref(params), f.ctypes.data_as(_I64P)), "batch_apply")
+        return f
+
+    def instances(self) -> int:
+        return int(_lib().dmf_engine_instances(self._h))
+
+    def flows(self) -> np.ndarray:
+        f = np.zeros(self.k, np.int64)
+        _check(_lib().dmf_batch_flows(self._h, f.ctypes.data_as(_I64P)), "batch_flows")
+        return f
+
+    def min_cut_all(self):
+        side = np.zeros(int(self.offsets[-1]), dtype=np.uint8)
+        _check(_lib().dmf_engine_min_cut(self._h, side.ctypes.data_as(C.POINTER(C.c_uint8))), "min_cut")
+        return [side[self.offsets[i]:self.offsets[i + 1]] for i in range(self.k)]
 
 
 def recalculate_max_flow(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="data"):

@@ -70,6 +70,10 @@ struct dmf_engine {
   int last_levels = 0;
   int last_variant = -1;          // variant of the last solve (-1 = none yet)
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
+  int32_t ninst = 1;              // independent instances stored as one disjoint union
+  std::vector<int64_t> voff;      // vertex offset of each instance (batched engines)
+  std::vector<int32_t> h_row;     // host copy of the CSR offsets (wide-list rebuilds)
+  int64_t wide_bytes = 0;
   uint64_t solve_id = 0;
   std::chrono::steady_clock::time_point t_solve;
 };
@@ -349,6 +353,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   if ((rc = reset_lists(E))) return rc;
   if ((rc = zero_dev(E, D.ctrl->flag, sizeof(D.ctrl->flag)))) return rc;
   if ((rc = zero_dev(E, &D.ctrl->flow, sizeof(long long)))) return rc;
+  if ((rc = zero_dev(E, D.iflow, sizeof(long long) * D.ninst))) return rc;
   LAUNCH(k_flow, blocks_for(n), D);
   E->last.rounds = round;
   E->last_variant = C.variant;
@@ -488,6 +493,189 @@ int ensure_update_capacity(dmf_engine* E, int32_t k) {
 
 }  // namespace
 
+namespace {
+
+// (Re)builds the wide-vertex structures for threshold wide_t: the static list
+// of every vertex of degree > wide_t with chunk bases, its bitmap, and the
+// chunk queues sized for it.
+int setup_wide(dmf_engine* E, int32_t wide_t) {
+  Dev& D = E->D;
+  std::vector<int32_t> wide_v, wide_b;
+  int64_t wide_chunks = 0;
+  for (int32_t v = 0; v < E->n; ++v) {
+    const int32_t d = E->h_row[v + 1] - E->h_row[v];
+    if (d > wide_t) {
+      wide_v.push_back(v);
+      wide_b.push_back(static_cast<int32_t>(wide_chunks));
+      wide_chunks += (d + kChunk - 1) / kChunk;
+    }
+  }
+  const size_t nwide = std::max<size_t>(1, wide_v.size());
+  std::vector<uint32_t> wbits((static_cast<size_t>(E->n) + 31) / 32 + 1, 0u);
+  for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
+  HIP_OK(hipStreamSynchronize(E->stream));
+  for (void* p : {static_cast<void*>(D.wq_vert), static_cast<void*>(D.wq_base),
+                  const_cast<void*>(static_cast<const void*>(D.wide_vert)),
+                  const_cast<void*>(static_cast<const void*>(D.wide_base)),
+                  const_cast<void*>(static_cast<const void*>(D.widebits))})
+    if (p) HIP_OK(hipFree(p));
+  E->bytes -= E->wide_bytes;
+  int32_t *wq_vert = nullptr, *wq_base = nullptr, *wide_vert = nullptr, *wide_base = nullptr;
+  uint32_t* widebits = nullptr;
+  const int64_t before = E->bytes;
+  int rc;
+  if ((rc = dalloc(E, &wq_vert, nwide * kNumQ)) || (rc = dalloc(E, &wq_base, nwide * kNumQ)) ||
+      (rc = dalloc(E, &wide_vert, nwide)) || (rc = dalloc(E, &wide_base, nwide)) ||
+      (rc = dalloc(E, &widebits, wbits.size())))
+    return rc;
+  E->wide_bytes = E->bytes - before;
+  if (!wide_v.empty()) {
+    HIP_OK(hipMemcpy(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size(), hipMemcpyHostToDevice));
+  }
+  HIP_OK(hipMemcpy(widebits, wbits.data(), sizeof(uint32_t) * wbits.size(), hipMemcpyHostToDevice));
+  D.wq_vert = wq_vert;
+  D.wq_base = wq_base;
+  D.wq_cap = static_cast<int32_t>(nwide);
+  D.wide_vert = wide_vert;
+  D.wide_base = wide_base;
+  D.widebits = widebits;
+  D.wide_t = wide_t;
+  D.wide_packed = (static_cast<unsigned long long>(wide_v.size()) << 32) |
+                  static_cast<unsigned long long>(wide_chunks);
+  return DMF_OK;
+}
+
+int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std::vector<int32_t>& snks,
+                  const std::vector<int32_t>* inst, int32_t ninst, dmf_engine** out) {
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DMF_EDEVICE;
+  dmf_engine* E = new (std::nothrow) dmf_engine();
+  if (!E) return DMF_ENOMEM;
+  E->n = g->n;
+  E->m = g->m;
+  E->s = srcs[0];
+  E->t = snks[0];
+  E->ninst = ninst;
+  E->h_row.assign(g->row, g->row + g->n + 1);
+  Dev& D = E->D;
+  D.n = g->n;
+  D.m = g->m;
+  D.s = srcs[0];
+  D.t = snks[0];
+  D.lane_max = kLaneMax;
+  D.nsrc = static_cast<int32_t>(srcs.size());
+  D.nsnk = static_cast<int32_t>(snks.size());
+  D.ninst = ninst;
+  const size_t n = g->n, m = g->m;
+  uint64_t hcap = 16;
+  while (hcap < 2 * static_cast<uint64_t>(m) + 16) hcap <<= 1;
+  std::vector<uint8_t> term(n, 0);
+  for (int32_t v : srcs) term[v] |= 1;
+  for (int32_t v : snks) term[v] |= 2;
+  int32_t *row, *col, *rev, *cap, *res, *rres, *height, *order, *wl, *wl2, *part, *owner, *hval;
+  int32_t *d_srcs, *d_snks, *d_inst = nullptr;
+  uint8_t* d_term;
+  long long* iflow;
+  unsigned long long* ctrs;
+  long long* excess;
+  uint8_t *side, *pfinal;
+  Ctrl* ctrl;
+  unsigned long long* hkey;
+  int rc = DMF_OK;
+  if ((rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
+      (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
+      (rc = dalloc(E, &rres, m)) || (rc = dalloc(E, &excess, n)) || (rc = dalloc(E, &height, n)) ||
+      (rc = dalloc(E, &order, n)) || (rc = dalloc(E, &wl, n)) || (rc = dalloc(E, &wl2, n)) ||
+      (rc = dalloc(E, &part, n)) || (rc = dalloc(E, &side, n)) || (rc = dalloc(E, &pfinal, n)) ||
+      (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &hkey, hcap)) || (rc = dalloc(E, &hval, hcap)) ||
+      (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
+      (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
+      (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
+      (inst && (rc = dalloc(E, &d_inst, n)))) {
+    dmf_engine_destroy(E);
+    return rc;
+  }
+  D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
+  D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
+  D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
+  D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
+  D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
+  {
+    const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
+    D.bottom_up = bu ? std::atoi(bu) : 0;
+    const char* co = std::getenv("DMF_BFS_COLLECT");
+    D.collect = co ? std::atoi(co) : 0;
+  }
+  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  E->own_stream = true;
+  if (hipHostMalloc(reinterpret_cast<void**>(&E->h_ctrl), sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  bool ok = hipMemcpyAsync(row, g->row, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(col, g->col, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(rev, g->rev, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(cap, g->cap, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(E->d_cap0, g->cap, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(d_term, term.data(), n, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(d_srcs, srcs.data(), sizeof(int32_t) * srcs.size(), hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            hipMemcpyAsync(d_snks, snks.data(), sizeof(int32_t) * snks.size(), hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+            (!inst || hipMemcpyAsync(d_inst, inst->data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, E->stream) == hipSuccess) &&
+            hipMemsetAsync(res, 0, sizeof(int32_t) * m, E->stream) == hipSuccess &&
+            hipMemsetAsync(rres, 0, sizeof(int32_t) * m, E->stream) == hipSuccess &&
+            hipMemsetAsync(excess, 0, sizeof(long long) * n, E->stream) == hipSuccess &&
+            hipMemsetAsync(height, 0, sizeof(int32_t) * n, E->stream) == hipSuccess &&
+            hipMemsetAsync(side, 0, n, E->stream) == hipSuccess &&
+            hipMemsetAsync(pfinal, 0, n, E->stream) == hipSuccess &&
+            hipMemsetAsync(ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess &&
+            hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
+            hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess &&
+            hipMemsetAsync(hval, 0xff, sizeof(int32_t) * hcap, E->stream) == hipSuccess;
+  if (!ok || setup_wide(E, kWideT) != DMF_OK) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  hipLaunchKernelGGL(k_build_hash, dim3(blocks_for(m)), dim3(kBlock), 0, E->stream, D);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E->stream) != hipSuccess) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  // Cooperative grid: as many blocks as can be co-resident.
+  int dev = 0, per_cu = 0;
+  hipDeviceProp_t prop{};
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+      prop.cooperativeLaunch &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_persistent),
+                                                   kBlock, 0) == hipSuccess &&
+      per_cu > 0) {
+    E->num_cu = prop.multiProcessorCount;
+    const char* capenv = std::getenv("DMF_COOP_BLOCKS_PER_CU");
+    const int want = capenv ? std::max(1, std::atoi(capenv)) : 8;
+    E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
+  }
+  // scheduler weights (compiled in) and state
+  if (hipMalloc(reinterpret_cast<void**>(&E->d_sched_w), sizeof(SchedWeights)) != hipSuccess ||
+      hipMalloc(reinterpret_cast<void**>(&E->d_sched), sizeof(SchedState)) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&E->h_sched), sizeof(SchedState), hipHostMallocDefault) != hipSuccess ||
+      hipMemcpy(E->d_sched_w, kSchedWeightsInit, sizeof(SchedWeights), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(E->d_sched, 0, sizeof(SchedState)) != hipSuccess) {
+    dmf_engine_destroy(E);
+    return DMF_EDEVICE;
+  }
+  E->bytes += sizeof(SchedWeights) + sizeof(SchedState);
+  if (const char* tp = std::getenv("DMF_TRACE")) E->trace = std::fopen(tp, "a");
+  *out = E;
+  return DMF_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int dmf_device_count(void) {
@@ -509,141 +697,46 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
   if (!g || !out || g->n < 2 || g->m < 0 || s < 0 || s >= g->n || t < 0 || t >= g->n || s == t)
     return DMF_EINVAL;
+  const std::vector<int32_t> srcs{s}, snks{t};
+  return create_engine(g, srcs, snks, nullptr, 1, out);
+}
+
+int dmf_batch_create(int32_t ninst, const dmf_csr* graphs, const int32_t* s, const int32_t* t,
+                     dmf_engine** out) {
+  if (!out || ninst < 1 || !graphs || !s || !t) return DMF_EINVAL;
   *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DMF_EDEVICE;
-  dmf_engine* E = new (std::nothrow) dmf_engine();
-  if (!E) return DMF_ENOMEM;
-  E->n = g->n;
-  E->m = g->m;
-  E->s = s;
-  E->t = t;
-  Dev& D = E->D;
-  D.n = g->n;
-  D.m = g->m;
-  D.s = s;
-  D.t = t;
-  const size_t n = g->n, m = g->m;
-  uint64_t hcap = 16;
-  while (hcap < 2 * static_cast<uint64_t>(m) + 16) hcap <<= 1;
-  // static list of wide vertices (degree > kWideT) with their chunk bases
-  std::vector<int32_t> wide_v, wide_b;
-  int64_t wide_chunks = 0;
-  for (int32_t v = 0; v < g->n; ++v) {
-    const int32_t d = g->row[v + 1] - g->row[v];
-    if (d > kWideT) {
-      wide_v.push_back(v);
-      wide_b.push_back(static_cast<int32_t>(wide_chunks));
-      wide_chunks += (d + kChunk - 1) / kChunk;
+  // disjoint union: instance i's vertices are shifted by voff[i], arcs by aoff[i]
+  std::vector<int64_t> voff(ninst + 1, 0), aoff(ninst + 1, 0);
+  for (int32_t i = 0; i < ninst; ++i) {
+    const dmf_csr& gi = graphs[i];
+    if (gi.n < 2 || gi.m < 0 || s[i] < 0 || s[i] >= gi.n || t[i] < 0 || t[i] >= gi.n || s[i] == t[i])
+      return DMF_EINVAL;
+    voff[i + 1] = voff[i] + gi.n;
+    aoff[i + 1] = aoff[i] + gi.m;
+  }
+  if (voff[ninst] > INT32_MAX - 1 || aoff[ninst] > INT32_MAX) return DMF_EINVAL;
+  const int32_t n = static_cast<int32_t>(voff[ninst]), m = static_cast<int32_t>(aoff[ninst]);
+  std::vector<int32_t> row(n + 1), col(m), cap(m), rev(m), inst(n), srcs(ninst), snks(ninst);
+  for (int32_t i = 0; i < ninst; ++i) {
+    const dmf_csr& gi = graphs[i];
+    const int32_t vo = static_cast<int32_t>(voff[i]), ao = static_cast<int32_t>(aoff[i]);
+    for (int32_t v = 0; v < gi.n; ++v) {
+      row[vo + v] = ao + gi.row[v];
+      inst[vo + v] = i;
     }
+    for (int32_t e = 0; e < gi.m; ++e) {
+      col[ao + e] = vo + gi.col[e];
+      cap[ao + e] = gi.cap[e];
+      rev[ao + e] = ao + gi.rev[e];
+    }
+    srcs[i] = vo + s[i];
+    snks[i] = vo + t[i];
   }
-  const size_t nwide = std::max<size_t>(1, wide_v.size());
-  int32_t *row, *col, *rev, *cap, *res, *rres, *height, *order, *wl, *wl2, *part, *owner, *hval;
-  int32_t *wq_vert, *wq_base, *wide_vert, *wide_base;
-  unsigned long long* ctrs;
-  uint32_t* widebits;
-  std::vector<uint32_t> wbits((n + 31) / 32 + 1, 0u);
-  for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
-  long long* excess;
-  uint8_t *side, *pfinal;
-  Ctrl* ctrl;
-  unsigned long long* hkey;
-  int rc = DMF_OK;
-  if ((rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
-      (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
-      (rc = dalloc(E, &rres, m)) || (rc = dalloc(E, &excess, n)) || (rc = dalloc(E, &height, n)) ||
-      (rc = dalloc(E, &order, n)) || (rc = dalloc(E, &wl, n)) || (rc = dalloc(E, &wl2, n)) ||
-      (rc = dalloc(E, &part, n)) || (rc = dalloc(E, &side, n)) || (rc = dalloc(E, &pfinal, n)) ||
-      (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &hkey, hcap)) || (rc = dalloc(E, &hval, hcap)) ||
-      (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &wq_vert, nwide * kNumQ)) ||
-      (rc = dalloc(E, &wq_base, nwide * kNumQ)) || (rc = dalloc(E, &wide_vert, nwide)) ||
-      (rc = dalloc(E, &wide_base, nwide)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
-      (rc = dalloc(E, &widebits, wbits.size()))) {
-    dmf_engine_destroy(E);
-    return rc;
-  }
-  D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
-  D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
-  D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
-  D.hmask = hcap - 1; D.owner = owner;
-  D.wq_vert = wq_vert; D.wq_base = wq_base; D.wq_cap = static_cast<int32_t>(nwide);
-  D.ctrs = ctrs;
-  D.widebits = widebits;
-  {
-    const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
-    D.bottom_up = bu ? std::atoi(bu) : 0;
-    const char* co = std::getenv("DMF_BFS_COLLECT");
-    D.collect = co ? std::atoi(co) : 0;
-  }
-  D.wide_vert = wide_vert; D.wide_base = wide_base;
-  D.wide_packed = (static_cast<unsigned long long>(wide_v.size()) << 32) |
-                  static_cast<unsigned long long>(wide_chunks);
-  if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
-  }
-  E->own_stream = true;
-  if (hipHostMalloc(reinterpret_cast<void**>(&E->h_ctrl), sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
-  }
-  bool ok = hipMemcpyAsync(row, g->row, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-            hipMemcpyAsync(col, g->col, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-            hipMemcpyAsync(rev, g->rev, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-            hipMemcpyAsync(cap, g->cap, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-            hipMemcpyAsync(E->d_cap0, g->cap, sizeof(int32_t) * m, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-            hipMemsetAsync(res, 0, sizeof(int32_t) * m, E->stream) == hipSuccess &&
-            hipMemsetAsync(rres, 0, sizeof(int32_t) * m, E->stream) == hipSuccess &&
-            hipMemsetAsync(excess, 0, sizeof(long long) * n, E->stream) == hipSuccess &&
-            hipMemsetAsync(height, 0, sizeof(int32_t) * n, E->stream) == hipSuccess &&
-            hipMemsetAsync(side, 0, n, E->stream) == hipSuccess &&
-            hipMemsetAsync(pfinal, 0, n, E->stream) == hipSuccess &&
-            hipMemsetAsync(ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess &&
-            hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
-            hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess &&
-            hipMemsetAsync(hval, 0xff, sizeof(int32_t) * hcap, E->stream) == hipSuccess &&
-            hipMemcpyAsync(widebits, wbits.data(), sizeof(uint32_t) * wbits.size(), hipMemcpyHostToDevice,
-                           E->stream) == hipSuccess &&
-            (wide_v.empty() ||
-             (hipMemcpyAsync(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size(), hipMemcpyHostToDevice,
-                             E->stream) == hipSuccess &&
-              hipMemcpyAsync(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size(), hipMemcpyHostToDevice,
-                             E->stream) == hipSuccess));
-  if (!ok) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
-  }
-  hipLaunchKernelGGL(k_build_hash, dim3(blocks_for(m)), dim3(kBlock), 0, E->stream, D);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E->stream) != hipSuccess) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
-  }
-  // Cooperative grid: as many blocks as can be co-resident.
-  int dev = 0, per_cu = 0;
-  hipDeviceProp_t prop{};
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-      prop.cooperativeLaunch &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_persistent),
-                                                   kBlock, 0) == hipSuccess &&
-      per_cu > 0) {
-    E->num_cu = prop.multiProcessorCount;
-    const char* cap = std::getenv("DMF_COOP_BLOCKS_PER_CU");
-    const int want = cap ? std::max(1, std::atoi(cap)) : 8;
-    E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
-  }
-  // scheduler weights (compiled in) and state
-  if (hipMalloc(reinterpret_cast<void**>(&E->d_sched_w), sizeof(SchedWeights)) != hipSuccess ||
-      hipMalloc(reinterpret_cast<void**>(&E->d_sched), sizeof(SchedState)) != hipSuccess ||
-      hipHostMalloc(reinterpret_cast<void**>(&E->h_sched), sizeof(SchedState), hipHostMallocDefault) != hipSuccess ||
-      hipMemcpy(E->d_sched_w, kSchedWeightsInit, sizeof(SchedWeights), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(E->d_sched, 0, sizeof(SchedState)) != hipSuccess) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
-  }
-  E->bytes += sizeof(SchedWeights) + sizeof(SchedState);
-  if (const char* tp = std::getenv("DMF_TRACE")) E->trace = std::fopen(tp, "a");
-  *out = E;
+  row[n] = m;
+  dmf_csr u{n, m, row.data(), col.data(), cap.data(), rev.data()};
+  int rc = create_engine(&u, srcs, snks, &inst, ninst, out);
+  if (rc) return rc;
+  (*out)->voff.assign(voff.begin(), voff.end());
   return DMF_OK;
 }
 
@@ -671,6 +764,11 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(D.hval);
   hipFree(D.owner);
   hipFree(D.ctrs);
+  hipFree(const_cast<uint8_t*>(D.term));
+  hipFree(const_cast<int32_t*>(D.srcs));
+  hipFree(const_cast<int32_t*>(D.snks));
+  hipFree(const_cast<int32_t*>(D.inst));
+  hipFree(D.iflow);
   hipFree(const_cast<uint32_t*>(D.widebits));
   hipFree(D.wq_vert);
   hipFree(D.wq_base);
@@ -756,6 +854,7 @@ int dmf_engine_apply_batch(dmf_engine* E, int32_t k, const int32_t* src, const i
 int dmf_engine_flow(dmf_engine* E, int64_t* flow) {
   if (!E || !flow) return DMF_EINVAL;
   HIP_OK(hipMemsetAsync(&E->D.ctrl->flow, 0, sizeof(long long), E->stream));
+  HIP_OK(hipMemsetAsync(E->D.iflow, 0, sizeof(long long) * E->D.ninst, E->stream));
   hipLaunchKernelGGL(k_flow, dim3(blocks_for(E->n)), dim3(kBlock), 0, E->stream, E->D);
   HIP_OK(hipGetLastError());
   int rc = sync_ctrl(E);
@@ -814,6 +913,81 @@ int dmf_engine_counters(dmf_engine* E, dmf_counters* c) {
 }
 
 int64_t dmf_engine_device_bytes(const dmf_engine* E) { return E ? E->bytes : 0; }
+
+int32_t dmf_engine_instances(const dmf_engine* E) { return E ? E->ninst : 0; }
+
+int dmf_batch_flows(dmf_engine* E, int64_t* flows) {
+  if (!E || !flows) return DMF_EINVAL;
+  int64_t total = 0;
+  int rc = dmf_engine_flow(E, &total);
+  if (rc) return rc;
+  HIP_OK(hipMemcpy(flows, E->D.iflow, sizeof(int64_t) * E->ninst, hipMemcpyDeviceToHost));
+  if (E->ninst == 1) flows[0] = total;
+  return DMF_OK;
+}
+
+int dmf_batch_static_solve(dmf_engine* E, const dmf_params* p, int64_t* flows) {
+  if (!E || !flows) return DMF_EINVAL;
+  int64_t total = 0;
+  int rc = dmf_engine_static_solve(E, p, &total);
+  if (rc) return rc;
+  return dmf_batch_flows(E, flows);
+}
+
+int dmf_batch_apply(dmf_engine* E, const int32_t* counts, const int32_t* src, const int32_t* dst,
+                    const int32_t* w, const dmf_params* p, int64_t* flows) {
+  if (!E || !counts || !flows) return DMF_EINVAL;
+  if (E->voff.empty()) E->voff = {0, E->n};
+  int64_t total = 0;
+  for (int32_t i = 0; i < E->ninst; ++i) {
+    if (counts[i] < 0) return DMF_EINVAL;
+    total += counts[i];
+  }
+  if (total > INT32_MAX || (total > 0 && (!src || !dst || !w))) return DMF_EINVAL;
+  std::vector<int32_t> gs(total), gd(total);
+  int64_t j = 0;
+  for (int32_t i = 0; i < E->ninst; ++i) {
+    const int64_t nv = E->voff[i + 1] - E->voff[i];
+    for (int32_t q = 0; q < counts[i]; ++q, ++j) {
+      if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv) return DMF_EINVAL;
+      gs[j] = static_cast<int32_t>(E->voff[i] + src[j]);
+      gd[j] = static_cast<int32_t>(E->voff[i] + dst[j]);
+    }
+  }
+  int64_t f = 0;
+  int rc = dmf_engine_apply_batch(E, static_cast<int32_t>(total), gs.data(), gd.data(), w, p, &f);
+  if (rc) return rc;
+  return dmf_batch_flows(E, flows);
+}
+
+int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, int32_t* wide_t) {
+  if (!E) return DMF_EINVAL;
+  static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {8, 64}};
+  double best_ms = 1e300;
+  int best = 1;
+  for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {
+    E->D.lane_max = cand[c][0];
+    int rc = setup_wide(E, cand[c][1]);
+    if (!rc) rc = dmf_engine_reset(E);
+    int64_t f = 0;
+    double ms = 0;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {  // first solve warms caches
+      rc = dmf_engine_static_solve(E, p, &f);
+      ms = E->last.device_ms;
+    }
+    if (rc) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  E->D.lane_max = cand[best][0];
+  int rc = setup_wide(E, cand[best][1]);
+  if (!rc) rc = dmf_engine_reset(E);
+  if (lane_max) *lane_max = cand[best][0];
+  if (wide_t) *wide_t = cand[best][1];
+  return rc;
+}
 
 int dmf_recalculate_max_flow(const dmf_csr* g, int32_t k, const int32_t* src, const int32_t* dst,
                              const int32_t* w, int32_t s, int32_t t, int32_t kc, int32_t variant,

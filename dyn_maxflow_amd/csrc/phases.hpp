@@ -4,15 +4,15 @@
 // kernel.  Included by engine.hip only.
 //
 // Work distribution (every phase that walks adjacency lists):
-//   * degree <= kLaneMax          one lane scans the vertex serially;
-//   * kLaneMax < degree <= kWideT one wave64 strides over the adjacency;
-//   * degree > kWideT             the vertex goes to a "wide queue" and its
+//   * degree <= lane_max          one lane scans the vertex serially;
+//   * lane_max < degree <= wide_t one wave64 strides over the adjacency;
+//   * degree > wide_t             the vertex goes to a "wide queue" and its
 //                                 adjacency is cut into kChunk-arc chunks that
 //                                 any wave of the grid picks up, so a hub of
 //                                 60k arcs is spread over ~240 waves instead
 //                                 of serialising one wave.
 // The BFS expands the non-wide part of a frontier kGroup vertices per wave
-// task as one flattened arc range (coalesced, balanced to within kGroup*kWideT
+// task as one flattened arc range (coalesced, balanced to within kGroup*wide_t
 // arcs per task).
 // A wide queue is a pair of arrays (vertex, first chunk) filled through one
 // 64-bit atomic that reserves the slot (high half) and the chunk range (low
@@ -34,8 +34,8 @@ namespace dmfdev {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kLaneMax = 32;
-constexpr int kWideT = 256;
+constexpr int kLaneMax = 32;   // default of Dev::lane_max (thresholds are tunable per engine)
+constexpr int kWideT = 256;    // default of Dev::wide_t
 constexpr int kChunk = 256;
 constexpr int kGroup = 16;      // frontier vertices per wave task in the flattened BFS expansion
 constexpr int kSlots = 64;       // work counters are spread over this many addresses
@@ -79,7 +79,19 @@ struct Ctrl {
 };
 
 struct Dev {
-  int32_t n, m, s, t;
+  int32_t n, m, s, t;  // s, t: the first source and sink (feature of the scheduler)
+  // terminals: bit 0 source, bit 1 sink.  One instance has one of each; a batch
+  // of independent instances stored as one disjoint union has one pair each.
+  const uint8_t* term;
+  const int32_t* srcs;
+  int32_t nsrc;
+  const int32_t* snks;
+  int32_t nsnk;
+  const int32_t* inst;  // instance of every vertex (nullptr: a single instance)
+  long long* iflow;     // per-instance flow
+  int32_t ninst;
+  int32_t lane_max;     // degree up to which one lane scans a vertex
+  int32_t wide_t;       // degree above which a vertex is split into chunks
   const int32_t* row;
   const int32_t* col;
   const int32_t* rev;
@@ -109,7 +121,7 @@ struct Dev {
   const int32_t* wide_vert;
   const int32_t* wide_base;
   unsigned long long wide_packed;
-  const uint32_t* widebits;  // bit v set when degree(v) > kWideT
+  const uint32_t* widebits;  // bit v set when degree(v) > wide_t
   // current update batch
   const int32_t* usrc;
   const int32_t* udst;
@@ -132,6 +144,10 @@ constexpr unsigned long long kEmptyKey = ~0ull;
 
 // ------------------------------------------------------------ small helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ bool is_src(const Dev& D, int v) { return D.term[v] & 1; }
+__device__ __forceinline__ bool is_snk(const Dev& D, int v) { return D.term[v] & 2; }
+__device__ __forceinline__ bool is_term(const Dev& D, int v) { return D.term[v] != 0; }
 
 __device__ __forceinline__ int ldr(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -359,8 +375,7 @@ __device__ void ph_init_residual(const Dev& D, const Range& r) {
 // K2 staticMaxFlow_kernel_7 (code/dyn_data.h:39-56), arc-parallel over s's
 // adjacency instead of one thread.  Each arc e of s has its own reverse arc, so
 // only the excess updates need atomics.
-__device__ void ph_saturate_source(const Dev& D, const Range& r, Tally& t) {
-  const int s = D.s;
+__device__ void saturate_one_source(const Dev& D, const Range& r, int s, Tally& t) {
   const int b = D.row[s], deg = D.row[s + 1] - b;
   long long sent = 0;
   for (int i = r.gid - lane_id(); i < deg; i += r.gsz) {  // wave-uniform loop
@@ -386,10 +401,12 @@ __device__ void ph_saturate_source(const Dev& D, const Range& r, Tally& t) {
   if (lane_id() == 0 && sent) add64(&D.excess[s], -sent);
 }
 
+__device__ void ph_saturate_source(const Dev& D, const Range& r, Tally& t) {
+  for (int si = 0; si < D.nsrc; ++si) saturate_one_source(D, r, D.srcs[si], t);
+}
 // Second half of the push-pull static saturation (code/dyn_pp.h:56-70): pull
 // every residual arc into t.
-__device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
-  const int v = D.t;
+__device__ void saturate_one_sink_pp(const Dev& D, const Range& r, int v, Tally& t) {
   const int b = D.row[v], deg = D.row[v + 1] - b;
   long long got = 0;
   for (int i = r.gid - lane_id(); i < deg; i += r.gsz) {
@@ -415,6 +432,10 @@ __device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
   if (lane_id() == 0 && got) add64(&D.excess[v], got);
 }
 
+__device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
+  for (int si = 0; si < D.nsnk; ++si) saturate_one_sink_pp(D, r, D.snks[si], t);
+}
+
 // ----------------------------------------------------------- global relabel
 // Root predicates of the three global relabels.
 enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2 };
@@ -433,16 +454,16 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
     if (v < n) {
       const long long ex = D.excess[v];
       if (mode == BFS_SINK) {
-        root = (v == D.t) || (v != D.s && ex < 0);
+        root = is_snk(D, v) || (!is_src(D, v) && ex < 0);
       } else {
         const bool inB = D.side[v] != 0;
-        root = (v == D.s) || (v == D.t) || (inB && ex < 0) || (!inB && ex > 0);
+        root = is_term(D, v) || (inB && ex < 0) || (!inB && ex > 0);
       }
       D.height[v] = root ? 0 : n;
       t.vert += 1;
       if (root) {
         const int d = degree(D, v);
-        if (d > kWideT) wq_append(D, 0, v, d);
+        if (d > D.wide_t) wq_append(D, 0, v, d);
       }
     }
     int rank;
@@ -464,7 +485,7 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
       t.vert += 1;
       if (root) {
         const int d = degree(D, v);
-        if (d > kWideT) wq_append(D, 0, v, d);
+        if (d > D.wide_t) wq_append(D, 0, v, d);
       }
     }
     int rank;
@@ -477,7 +498,7 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
 __device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int e, int w) {
   if (mode == BFS_SINK) return D.rres[e] > 0;
   if (mode == BFS_PP) {
-    if (w == D.s || w == D.t) return false;
+    if (is_term(D, w)) return false;
     return vB ? (D.rres[e] > 0) : (D.res[e] > 0);
   }
   // BFS_FINAL: non-B neighbours of the stranded part only
@@ -545,7 +566,7 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
       const int v = D.order[lo + i];
       b = D.row[v];
       deg = D.row[v + 1] - b;
-      if (deg > kWideT) deg = 0;  // handled through its chunks
+      if (deg > D.wide_t) deg = 0;  // handled through its chunks
       vB = (mode == BFS_PP && D.side[v] != 0) ? 1 : 0;
       t.vert += 1;
     }
@@ -638,7 +659,7 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
       t.vert += 1;
     }
     bool hit = false;
-    unsigned long long big = __ballot(cand && deg > kLaneMax);
+    unsigned long long big = __ballot(cand && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
@@ -656,7 +677,7 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
       }
       if (lane_id() == leader) hit = found;
     }
-    if (cand && deg <= kLaneMax) {
+    if (cand && deg <= D.lane_max) {
       for (int e = b; e < b + deg; ++e) {
         t.arc += 1;
         if (D.res[e] > 0 && D.height[D.col[e]] == lvl) {
@@ -667,7 +688,7 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
     }
     if (hit) {
       D.height[w] = lvl + 1;
-      if (deg > kWideT) wq_append(D, qnext, w, deg);
+      if (deg > D.wide_t) wq_append(D, qnext, w, deg);
     }
     st.push(hit, w, next_cnt, out);
   }
@@ -685,11 +706,11 @@ __device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
     int v = -1;
     if (i < cnt) {
       v = D.order[i];
-      act = v != D.s && v != D.t && D.excess[v] > 0 && D.height[v] < D.n;
+      act = !is_term(D, v) && D.excess[v] > 0 && D.height[v] < D.n;
       t.vert += 1;
       if (act) {
         const int d = degree(D, v);
-        if (d > kWideT) {
+        if (d > D.wide_t) {
           wq_append(D, kQActive, v, d);
           act = false;
         }
@@ -710,7 +731,7 @@ __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) 
     int v = -1;
     if (i < cnt) {
       v = D.order[i];
-      if (v != D.s && v != D.t && D.height[v] < D.n) {
+      if (!is_term(D, v) && D.height[v] < D.n) {
         const long long ex = D.excess[v];
         if (D.side[v]) a = ex > 0;
         else b = ex < 0;
@@ -718,7 +739,7 @@ __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) 
       t.vert += 1;
       if (a || b) {
         const int d = degree(D, v);
-        if (d > kWideT) {
+        if (d > D.wide_t) {
           wq_append(D, a ? kQActive : kQPull, v, d);
           a = b = false;
         }
@@ -744,7 +765,7 @@ __device__ void ph_build_final_wl(const Dev& D, const Range& r, int cnt, Tally& 
       t.vert += 1;
       if (act) {
         const int d = degree(D, v);
-        if (d > kWideT) {
+        if (d > D.wide_t) {
           wq_append(D, kQActive, v, d);
           act = false;
         }
@@ -766,7 +787,7 @@ __device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
   return true;
 }
 
-// Lane path (degree <= kLaneMax): up to kc discharge steps.  A step scans the
+// Lane path (degree <= lane_max): up to kc discharge steps.  A step scans the
 // arcs once and pushes along every residual arc to a lower neighbour (the arcs
 // the reference's step of code/dyn_data.h:94-129 may use, one per scan there)
 // until the excess is spent; if excess remains, v relabels to (lowest residual
@@ -950,7 +971,7 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
     hv = __shfl(hv, 0);
     const long long amt = PULL ? -ex : ex;
     bool act = amt > 0 && (PULL || hv < n);
-    if (!list) act = act && v != D.s && v != D.t;
+    if (!list) act = act && !is_term(D, v);
     if (!act) continue;
     if (!list && lane_id() == 0 && *flag == 0) *flag = 1;
     int mo;
@@ -964,13 +985,13 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
       b = D.row[v];
       deg = D.row[v + 1] - b;
       if (!list) {
-        const bool act = deg <= kWideT && v != D.s && v != D.t && D.excess[v] > 0 && D.height[v] < n;
+        const bool act = deg <= D.wide_t && !is_term(D, v) && D.excess[v] > 0 && D.height[v] < n;
         if (!act) v = -1;
         else if (*flag == 0) *flag = 1;
       }
       if (v >= 0) t.vert += 1;
     }
-    unsigned long long big = __ballot(v >= 0 && deg > kLaneMax);
+    unsigned long long big = __ballot(v >= 0 && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
@@ -979,7 +1000,7 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
       const int bd = __shfl(deg, leader);
       wave_push_cycles<PULL>(D, bv, bb, bd, kc, pmode, t);
     }
-    if (v >= 0 && deg <= kLaneMax) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
+    if (v >= 0 && deg <= D.lane_max) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
   }
 }
 
@@ -1049,13 +1070,13 @@ __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int
       v = list ? list[i] : i;
       b = D.row[v];
       deg = D.row[v + 1] - b;
-      if (deg > kWideT) v = -1;  // wide: through W
+      if (deg > D.wide_t) v = -1;  // wide: through W
       else {
         h = D.height[v];
         t.vert += 1;
       }
     }
-    unsigned long long big = __ballot(v >= 0 && deg > kLaneMax);
+    unsigned long long big = __ballot(v >= 0 && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
@@ -1065,7 +1086,7 @@ __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int
       const int bh = __shfl(h, leader);
       for (int e = bb + lane_id(); e < bb + bd; e += kWave) repair_one(D, rmode, bv, bh, e, t);
     }
-    if (v >= 0 && deg <= kLaneMax)
+    if (v >= 0 && deg <= D.lane_max)
       for (int e = b; e < b + deg; ++e) repair_one(D, rmode, v, h, e, t);
   }
 }
@@ -1123,10 +1144,10 @@ __device__ void ph_pp_cut(const Dev& D, const Range& r, Tally& t) {
       b = D.row[v];
       deg = D.row[v + 1] - b;
       self = D.side[v];
-      mine = deg <= kWideT;
+      mine = deg <= D.wide_t;
       t.vert += 1;
     }
-    unsigned long long big = __ballot(mine && deg > kLaneMax);
+    unsigned long long big = __ballot(mine && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
@@ -1142,7 +1163,7 @@ __device__ void ph_pp_cut(const Dev& D, const Range& r, Tally& t) {
       acc = wave_sum_i64(acc);
       if (lane_id() == 0) D.excess[bv] = acc;
     }
-    if (mine && deg <= kLaneMax) {
+    if (mine && deg <= D.lane_max) {
       long long acc = 0;
       for (int e = b; e < b + deg; ++e) acc += pp_arc_fix(D, e, self);
       t.arc += deg;
@@ -1188,8 +1209,11 @@ __device__ void ph_flow(const Dev& D, const Range& r) {
   long long acc = 0;
   for (int v = r.gid; v < D.n; v += r.gsz) {
     const long long ex = D.excess[v];
-    if (v == D.t) acc += ex;
-    if (ex < 0 && v != D.s) acc += ex;
+    long long c = 0;
+    if (is_snk(D, v)) c += ex;
+    if (ex < 0 && !is_src(D, v)) c += ex;
+    if (c && D.inst) add64(&D.iflow[D.inst[v]], c);
+    acc += c;
   }
   acc = wave_sum_i64(acc);
   if (lane_id() == 0 && acc) add64(&D.ctrl->flow, acc);
@@ -1308,7 +1332,7 @@ __device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark) {
     const int v = base + lane_id();
     bool root = false;
     if (v < D.n) {
-      root = v == D.s || (v != D.t && D.excess[v] > 0);
+      root = is_src(D, v) || (!is_snk(D, v) && D.excess[v] > 0);
       mark[v] = root ? 1 : 0;
     }
     int rank;
@@ -1328,7 +1352,7 @@ __device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int lo
       b = D.row[v];
       deg = D.row[v + 1] - b;
     }
-    unsigned long long big = __ballot(deg > kLaneMax);
+    unsigned long long big = __ballot(deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
@@ -1348,7 +1372,7 @@ __device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int lo
         if (hit) out[off + rank] = w;
       }
     }
-    const int my_deg = deg <= kLaneMax ? deg : 0;
+    const int my_deg = deg <= D.lane_max ? deg : 0;
     int maxd = my_deg;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
@@ -1729,6 +1753,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
       G.sync();
     }
   }
+  for (int i = r.gid; i < D.ninst; i += r.gsz) D.iflow[i] = 0;
   G.sync();
   if (is_block0_thread0()) {
     D.ctrl->wl_cnt = D.ctrl->wl2_cnt = D.ctrl->part_cnt = 0;

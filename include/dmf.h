@@ -210,6 +210,29 @@ int dmf_engine_counters(dmf_engine* e, dmf_counters* c);
 /* Bytes of device memory held by this engine. */
 int64_t dmf_engine_device_bytes(const dmf_engine* e);
 
+/* Tries a few (lane_max, wide_t) degree-bucket thresholds -- the degree up to
+ * which one lane scans a vertex, and above which a vertex is split into chunks
+ * spread over many waves -- with a static solve each, keeps the fastest and
+ * resets the engine.  No reference counterpart (its kernels use one thread per
+ * vertex). */
+int dmf_engine_autotune(dmf_engine* e, const dmf_params* p, int32_t* lane_max, int32_t* wide_t);
+
+/* ---------------------------------------------------- batched instances */
+/* Many independent (graph, s, t) instances kept resident as one disjoint union
+ * and solved together: every phase of a solve covers all instances at once
+ * (with the persistent driver, one launch per batch for all of them).
+ * No reference counterpart (the reference solves one graph per process). */
+int dmf_batch_create(int32_t ninst, const dmf_csr* graphs, const int32_t* s, const int32_t* t,
+                     dmf_engine** out);
+int32_t dmf_engine_instances(const dmf_engine* e);
+/* flows[ninst] receives each instance's max-flow value. */
+int dmf_batch_static_solve(dmf_engine* e, const dmf_params* p, int64_t* flows);
+/* counts[i] update records for instance i, concatenated in instance order in
+ * src/dst/w with instance-local vertex ids. */
+int dmf_batch_apply(dmf_engine* e, const int32_t* counts, const int32_t* src, const int32_t* dst,
+                    const int32_t* w, const dmf_params* p, int64_t* flows);
+int dmf_batch_flows(dmf_engine* e, int64_t* flows);
+
 /* ------------------------------------------ reference in-process entry points */
 /* int recalculate_max_flow(graph&, std::vector<update>, int batchSize, int s,
  * int t, int kernel_cycles)  (code/dyn_data_bench.cu:171-298,

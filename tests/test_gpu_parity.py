@@ -231,3 +231,61 @@ def test_learned_scheduler_keeps_results(gpu, tmp_path):
     assert any(r["what"] == "round" for r in recs)
     assert sum(r["what"] == "summary" for r in recs) == 5
     assert all(len(r["feat"]) == 8 for r in recs if r["what"] != "summary")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_batched_instances_match_solo_solves(gpu, mode):
+    """Independent instances solved together (one disjoint union, one launch per
+    phase for all of them): every instance's flow and S_min equal its solo
+    engine's and Dinic's after every batch."""
+    dmf = gpu
+    rng = np.random.default_rng(4)
+    cases = []
+    for i in range(12):
+        scale = int(rng.integers(7, 12))
+        g, s, t, (u, v, w) = rmat_case(scale, int(rng.integers(8, 33)), 40 + i)
+        cases.append((g, s, t, u, v, w))
+    p = dmf.default_params(cases[0][0], mode=mode)
+    say(f"[batched] mode={mode} instances={len(cases)}")
+    solos = [dmf.Engine(c[0], c[1], c[2]) for c in cases]
+    try:
+        with dmf.BatchEngine([c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases]) as be:
+            assert be.instances() == len(cases)
+            flows = be.static_solve_all(p)
+            for i, (g, s, t, u, v, w) in enumerate(cases):
+                assert flows[i] == oracle.dinic(g, s, t)[0] == solos[i].static_solve(p), i
+            curs = [c[5].copy() for c in cases]
+            applied = [[] for _ in cases]
+            for b in range(3):
+                ups = []
+                for i, (g, s, t, u, v, w) in enumerate(cases):
+                    k = max(1, len(u) * (b + 1) // 50)
+                    ups.append(dmf.gen_updates(u, v, curs[i], k, 600 + 10 * i + b))
+                    applied[i].append(ups[-1])
+                say(f"[batched] mode={mode} batch {b}")
+                flows = be.apply_all(ups, p)
+                assert np.array_equal(flows, be.flows())
+                sides = be.min_cut_all()
+                for i, (g, s, t, u, v, w) in enumerate(cases):
+                    fd, side = oracle.dinic(g, s, t, cap=caps_after(g, applied[i]))
+                    f_solo = solos[i].apply_batch(*ups[i], p)
+                    assert flows[i] == fd == f_solo, (b, i)
+                    assert np.array_equal(sides[i], side), (b, i)
+    finally:
+        for e in solos:
+            e.close()
+
+
+def test_autotune_keeps_results(gpu):
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 10)
+    with dmf.Engine(g, s, t) as eng:
+        p = dmf.default_params(g)
+        lm, wt = eng.autotune(p)
+        assert lm > 0 and wt > 0
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "autotuned static")
+        cur = w.copy()
+        src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 25, 77)
+        f = eng.apply_batch(src, dst, nw, p)
+        check_engine(eng, g, caps_after(g, [(src, dst, nw)]), s, t, f, "autotuned batch")

@@ -120,6 +120,9 @@ def _lib():
         "dmf_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                    C.c_uint64, C.c_int32, C.c_int32, _I64P, C.POINTER(_I32P),
                                    C.POINTER(_I32P), C.POINTER(_I32P)]),
+        "dmf_gen_rmat_n": (C.c_int, [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint64, C.c_int32, C.c_int32, _I64P, C.POINTER(_I32P),
+                                     C.POINTER(_I32P), C.POINTER(_I32P)]),
         "dmf_gen_grid": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64, C.c_int32, C.c_int32, _I64P,
                                    C.POINTER(_I32P), C.POINTER(_I32P), C.POINTER(_I32P)]),
         "dmf_pick_source_sink": (C.c_int, [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, _I32P]),
@@ -264,6 +267,15 @@ def gen_rmat(scale: int, edge_factor: int, seed: int, a=0.57, b=0.19, c=0.19, ca
     u, v, w = _I32P(), _I32P(), _I32P()
     _check(_lib().dmf_gen_rmat(scale, edge_factor, a, b, c, seed, cap_lo, cap_hi, C.byref(k),
                                C.byref(u), C.byref(v), C.byref(w)), "gen_rmat")
+    return _take_i32(u, k.value), _take_i32(v, k.value), _take_i32(w, k.value)
+
+
+def gen_rmat_n(n: int, samples: int, seed: int, a=0.57, b=0.19, c=0.19, cap_lo=1, cap_hi=100):
+    """R-MAT over any vertex count n (dmf_gen_rmat_n)."""
+    k = C.c_int64()
+    u, v, w = _I32P(), _I32P(), _I32P()
+    _check(_lib().dmf_gen_rmat_n(n, samples, a, b, c, seed, cap_lo, cap_hi, C.byref(k),
+                                 C.byref(u), C.byref(v), C.byref(w)), "gen_rmat_n")
     return _take_i32(u, k.value), _take_i32(v, k.value), _take_i32(w, k.value)
 
 

@@ -4,6 +4,7 @@ Produces, next to this file:
   libdmf.so                 host formats/tools + HIP engine (C-ABI of include/dmf.h)
   bin/dyn_{data,pp,topo}_bench{,_large}   drivers with the reference's argv/stdout
   bin/dmf_input             seeded input tool (graph / update .bin writer)
+  bin/dmf_multi_bench       many independent instances solved as one batched engine
 
 Everything is compiled for gfx950 with hipcc; packed-fp32 instruction
 selection is disabled for all device code.
@@ -82,6 +83,10 @@ def build(verbose: bool = False) -> str:
             _run(["g++", *HOST_FLAGS, "-I", INCLUDE, f"-DDMF_BENCH_VARIANT=\"{variant}\"",
                   f"-DDMF_BENCH_LARGE={large}", drv, "-o", os.path.join(BIN, name),
                   "-L", HERE, "-ldmf", f"-Wl,-rpath,$ORIGIN/.."])
+    multi = os.path.join(CSRC, "multi_bench.cpp")
+    if os.path.exists(multi):
+        _run(["g++", *HOST_FLAGS, "-I", INCLUDE, multi, "-o", os.path.join(BIN, "dmf_multi_bench"),
+              "-L", HERE, "-ldmf", "-pthread", f"-Wl,-rpath,$ORIGIN/.."])
     tool = os.path.join(CSRC, "input_tool.cpp")
     if os.path.exists(tool):
         _run(["g++", *HOST_FLAGS, "-I", INCLUDE, tool, "-o", os.path.join(BIN, "dmf_input"),

@@ -76,6 +76,14 @@ struct dmf_engine {
   int64_t wide_bytes = 0;
   uint64_t solve_id = 0;
   std::chrono::steady_clock::time_point t_solve;
+  // graph-replayed BFS (mode 2): segments of 2^i level pairs per BFS mode
+  static constexpr int kSegSizes = 7;
+  Dev* d_dev = nullptr;
+  hipGraph_t seg_graph[3][kSegSizes] = {};
+  hipGraphExec_t seg_exec[3][kSegSizes] = {};
+  bool use_graph = false;
+  bool graph_failed = false;
+  int seg_levels[3] = {0, 0, 0};
 };
 
 namespace {
@@ -175,6 +183,96 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
   return DMF_OK;
 }
 
+constexpr int kNoGraph = 1;  // internal: capture refused, use per-level launches
+
+// Captures a segment of 2^idx (level, advance) pairs for BFS mode `mode`.
+int capture_segment(dmf_engine* E, int mode, int idx) {
+  if (E->seg_exec[mode][idx]) return DMF_OK;
+  const int grid = std::min(phase_blocks(E->n, chunks_of(E->D.wide_packed)), std::max(1, E->num_cu) * 8);
+  hipStream_t st = E->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
+  bool ok = true;
+  for (int i = 0; i < (1 << idx) && ok; ++i) {
+    hipLaunchKernelGGL(k_bfs_level_dev, dim3(grid), dim3(kBlock), 0, st, E->d_dev, mode);
+    ok = hipGetLastError() == hipSuccess;
+    hipLaunchKernelGGL(k_bfs_advance, dim3(1), dim3(kWave), 0, st, E->d_dev);
+    ok = ok && hipGetLastError() == hipSuccess;
+  }
+  hipGraph_t g = nullptr;
+  const bool ended = hipStreamEndCapture(st, &g) == hipSuccess;
+  if (!ok || !ended || !g) {
+    if (g) hipGraphDestroy(g);
+    return kNoGraph;
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return kNoGraph;
+  }
+  E->seg_graph[mode][idx] = g;
+  E->seg_exec[mode][idx] = x;
+  return DMF_OK;
+}
+
+void destroy_segments(dmf_engine* E) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < dmf_engine::kSegSizes; ++b) {
+      if (E->seg_exec[a][b]) hipGraphExecDestroy(E->seg_exec[a][b]);
+      if (E->seg_graph[a][b]) hipGraphDestroy(E->seg_graph[a][b]);
+      E->seg_exec[a][b] = nullptr;
+      E->seg_graph[a][b] = nullptr;
+    }
+}
+
+// Global relabel with the level loop replayed from captured graphs: one host
+// check per segment instead of one per level.  The first segment is as long as
+// the previous BFS of this mode was deep; later ones grow.  Same result as
+// relabel_stepped (the same level kernel body runs on the same frontiers).
+int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
+  Dev& D = E->D;
+  int rc;
+  if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) return rc;
+  if ((rc = zero_dev(E, D.ctrl->wq, 3 * sizeof(unsigned long long)))) return rc;
+  if ((rc = zero_dev(E, &D.ctrl->blo, 2 * sizeof(int32_t)))) return rc;
+  hipLaunchKernelGGL(k_store_dev, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
+  if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
+  else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode);
+  int idx = 0;
+  while (idx + 1 < dmf_engine::kSegSizes && (1 << idx) < E->seg_levels[mode] + 1) ++idx;
+  int lvl = 0;
+  for (;;) {
+    if ((rc = capture_segment(E, mode, idx))) return rc;
+    HIP_OK(hipGraphLaunch(E->seg_exec[mode][idx], E->stream));
+    ++E->launches;
+    if ((rc = sync_ctrl(E))) return rc;
+    lvl = E->h_ctrl->blvl;
+    if (E->h_ctrl->qcnt[lvl % 3] == 0) break;
+    if (lvl > E->n + 1) return DMF_EDEVICE;
+    if (idx + 1 < dmf_engine::kSegSizes) ++idx;
+  }
+  E->seg_levels[mode] = lvl;
+  E->last.bfs_levels += lvl;
+  E->last.global_relabels += 1;
+  E->last_levels = lvl;
+  *visited = E->h_ctrl->blo;
+  return DMF_OK;
+}
+
+int relabel(dmf_engine* E, int mode, int part_cnt, int* visited) {
+  const Dev& D = E->D;
+  if (E->use_graph && !E->graph_failed && D.bottom_up == 0 && D.collect == 0) {
+    const int rc = relabel_graph(E, mode, part_cnt, visited);
+    if (rc != kNoGraph) return rc;
+    // capture or instantiation refused: keep the host-driven level loop for
+    // this engine; relabel_stepped starts the BFS over from its roots
+    E->graph_failed = true;
+    std::fprintf(stderr, "dmf: BFS graph capture unavailable, using per-level launches\n");
+  }
+  return relabel_stepped(E, mode, part_cnt, visited);
+}
+
 int reset_lists(dmf_engine* E) {
   Dev& D = E->D;
   int rc;
@@ -261,7 +359,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       bool fresh = false;
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
-        if ((rc = relabel_stepped(E, BFS_SINK, 0, &visited))) return rc;
+        if ((rc = relabel(E, BFS_SINK, 0, &visited))) return rc;
         since = 0;
         fresh = true;
       }
@@ -275,7 +373,8 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         wpk = E->h_ctrl->wq[kQActive];
         if (cnt == 0 && wpk == 0) {
           if (fresh) break;
-          since = C.interval;
+          since = C.interval;  // confirm convergence on fresh labels
+          want_relabel = true;
           continue;
         }
       }
@@ -303,15 +402,29 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     // every solve leaves a cut in side[] so that a push-pull batch may follow
     if (conv) LAUNCH(k_find_cut, blocks_for(n), D);
   } else {
+    // push-pull batch; stale two-sided labels stay valid between relabels and
+    // convergence is only accepted on fresh ones (part_final reads them)
+    int since = C.interval, visited = 0;
+    bool want_relabel = true;
     for (;;) {
-      int visited = 0;
-      if ((rc = relabel_stepped(E, BFS_PP, 0, &visited))) return rc;
+      bool fresh = false;
+      const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
+      if (due) {
+        if ((rc = relabel(E, BFS_PP, 0, &visited))) return rc;
+        since = 0;
+        fresh = true;
+      }
       if ((rc = reset_lists(E))) return rc;
       LAUNCH(k_build_pp_wl, blocks_for(visited), D, visited);
       if ((rc = sync_ctrl(E))) return rc;
       const int np = E->h_ctrl->wl_cnt, nq = E->h_ctrl->wl2_cnt;
       const unsigned long long wp = E->h_ctrl->wq[kQActive], wq = E->h_ctrl->wq[kQPull];
-      if (np + nq == 0 && wp == 0 && wq == 0) break;
+      if (np + nq == 0 && wp == 0 && wq == 0) {
+        if (fresh) break;
+        since = C.interval;
+        want_relabel = true;
+        continue;
+      }
       if (round >= C.max_rounds) {
         conv = false;
         break;
@@ -321,6 +434,11 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       if (np || wp) LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
       if (nq || wq) LAUNCH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
       ++round;
+      ++since;
+      if (consult) {
+        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0))) return rc;
+        want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
+      }
     }
     if (conv) {
       if ((rc = reset_lists(E))) return rc;
@@ -330,7 +448,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       if (parts > 0) {
         for (;;) {
           int visited = 0;
-          if ((rc = relabel_stepped(E, BFS_FINAL, parts, &visited))) return rc;
+          if ((rc = relabel(E, BFS_FINAL, parts, &visited))) return rc;
           if ((rc = zero_dev(E, &D.ctrl->wl_cnt, sizeof(int32_t)))) return rc;
           if ((rc = zero_dev(E, &D.ctrl->wq[kQActive], sizeof(unsigned long long)))) return rc;
           LAUNCH(k_build_final_wl, blocks_for(parts), D, parts);
@@ -453,8 +571,9 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   ++E->solve_id;
   E->t_solve = std::chrono::steady_clock::now();
   // the scheduler runs between rounds on the host-driven path
-  const bool stepped = (p && p->mode == 1) || E->coop_blocks <= 0 || C.scheduler == DMF_SCHED_LEARNED ||
-                       E->trace != nullptr;
+  const bool stepped = (p && (p->mode == 1 || p->mode == 2)) || E->coop_blocks <= 0 ||
+                       C.scheduler == DMF_SCHED_LEARNED || E->trace != nullptr;
+  E->use_graph = p && p->mode == 2;
   int rc_driver = stepped ? solve_stepped(E, C) : solve_persistent(E, C);
   if (rc_driver == DMF_EDEVICE) return rc_driver;
   HIP_OK(hipEventRecord(E->ev1, E->stream));
@@ -614,7 +733,8 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
   }
   E->own_stream = true;
   if (hipHostMalloc(reinterpret_cast<void**>(&E->h_ctrl), sizeof(Ctrl), hipHostMallocDefault) != hipSuccess ||
-      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess) {
+      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess ||
+      dalloc(E, &E->d_dev, 1) != DMF_OK) {
     dmf_engine_destroy(E);
     return DMF_EDEVICE;
   }
@@ -778,6 +898,8 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(E->d_udst);
   hipFree(E->d_uw);
   hipFree(E->d_uarc);
+  destroy_segments(E);
+  hipFree(E->d_dev);
   if (E->h_ctrl) hipHostFree(E->h_ctrl);
   if (E->h_sched) hipHostFree(E->h_sched);
   hipFree(E->d_sched_w);

@@ -357,10 +357,21 @@ int dmf_write_updates_bin(const char* path, int32_t k, const int32_t* src, const
 int dmf_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
                  uint64_t seed, int32_t cap_lo, int32_t cap_hi, int64_t* k, int32_t** u,
                  int32_t** v, int32_t** w) {
-  if (scale < 1 || scale > 30 || edge_factor < 1 || !k || !u || !v || !w || cap_lo > cap_hi ||
+  if (scale < 1 || scale > 30 || edge_factor < 1) return DMF_EINVAL;
+  return dmf_gen_rmat_n(int32_t{1} << scale, (int64_t{1} << scale) * edge_factor, a, b, c, seed, cap_lo,
+                        cap_hi, k, u, v, w);
+}
+
+// R-MAT over [0, n): samples are drawn on the enclosing power of two and those
+// with an endpoint >= n are dropped like self-loops and duplicates, so for a
+// power-of-two n the output equals dmf_gen_rmat's.
+int dmf_gen_rmat_n(int32_t n, int64_t samples, double a, double b, double c, uint64_t seed,
+                   int32_t cap_lo, int32_t cap_hi, int64_t* k, int32_t** u, int32_t** v, int32_t** w) {
+  if (n < 2 || n > (int32_t{1} << 30) || samples < 1 || !k || !u || !v || !w || cap_lo > cap_hi ||
       a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
     return DMF_EINVAL;
-  const int64_t samples = (int64_t{1} << scale) * edge_factor;
+  int32_t scale = 1;
+  while ((int64_t{1} << scale) < n) ++scale;
   Xoshiro256ss rng(seed);
   PairMap seen(samples);
   int32_t* U = alloc_array<int32_t>(samples);
@@ -389,7 +400,7 @@ int dmf_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double 
       x = (x << 1) | bx;
       y = (y << 1) | by;
     }
-    if (x == y) continue;
+    if (x == y || x >= static_cast<uint32_t>(n) || y >= static_cast<uint32_t>(n)) continue;
     bool ins = false;
     seen.find_or_insert(pair_key(static_cast<int32_t>(x), static_cast<int32_t>(y)), &ins);
     if (!ins) continue;

@@ -73,7 +73,8 @@ struct Ctrl {
   int32_t visited;    // vertices labelled by the last global relabel
   int32_t rounds;     // rounds of the last solve
   int32_t inc;        // updates that raise an arc's capacity (scheduler feature)
-  int32_t pad[2];
+  int32_t blo;        // graph-replayed BFS: first order[] index of the current level
+  int32_t blvl;       // graph-replayed BFS: current level
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
   long long flow;
 };
@@ -1428,6 +1429,40 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
   }
   t.flush(D.ctrs);
 }
+// Graph-replayed BFS levels.  A captured segment is a fixed chain of
+// (k_bfs_level_dev, k_bfs_advance) pairs; the frontier of the current level is
+// found from the device cursor (ctrl->blo, ctrl->blvl) rather than from kernel
+// arguments, and once a level comes out empty every later pair of the segment
+// returns at once.  The engine state is read through a device copy of Dev
+// (k_store_dev) because captured kernel arguments are frozen at capture time.
+__global__ void k_store_dev(Dev D, Dev* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = D;
+}
+__global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, int mode) {
+  const Dev D = *pD;
+  const int lvl = ldr(&D.ctrl->blvl);
+  const int cnt = ldr(&D.ctrl->qcnt[lvl % 3]);
+  if (cnt == 0) return;  // uniform: the BFS is over
+  const int lo = ldr(&D.ctrl->blo);
+  Tally t;
+  const WList W = wlist(D, lvl % 3, ldru64(&D.ctrl->wq[lvl % 3]));
+  ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+  t.flush(D.ctrs);
+}
+// One thread: moves the cursor past the level just expanded.  Slot (lvl+2)%3
+// held the frontier size of level lvl-1 (already folded into blo) and is the
+// one level lvl+1 appends to.
+__global__ void k_bfs_advance(const Dev* __restrict__ pD) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Ctrl* c = pD->ctrl;
+  const int lvl = c->blvl;
+  const int cnt = c->qcnt[lvl % 3];
+  if (cnt == 0) return;
+  c->qcnt[(lvl + 2) % 3] = 0;
+  c->wq[(lvl + 2) % 3] = 0;
+  c->blo += cnt;
+  c->blvl = lvl + 1;
+}
 __global__ void __launch_bounds__(kBlock) k_bfs_collect(Dev D, int lo, int cnt, int lvl) {
   Tally t;
   ph_bfs_collect(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
@@ -1694,15 +1729,30 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
       G.sync();
     }
   } else {
-    // ---- push-pull batch: part_pp (code/dyn_pp_bench.cu:6-88)
+    // ---- push-pull batch: part_pp (code/dyn_pp_bench.cu:6-88).  The
+    // reference relabels every round; with a longer interval the labels of the
+    // last two-sided BFS stay valid (pushes and pulls relabel locally and the
+    // repair passes restore validity), and convergence is only accepted on
+    // fresh labels, so the final BFS that part_final reads is always current.
+    int since = C.interval, visited = 0;
     for (;;) {
-      const int visited = relabel_persistent(D, G, r, BFS_PP, 0, t);
+      bool fresh = false;
+      if (since >= C.interval) {
+        visited = relabel_persistent(D, G, r, BFS_PP, 0, t);
+        since = 0;
+        fresh = true;
+      }
       ph_build_pp_wl(D, r, visited, t);
       G.sync();
       const int np = ldr(&D.ctrl->wl_cnt), nq = ldr(&D.ctrl->wl2_cnt);
       const unsigned long long wp = ldru64(&D.ctrl->wq[kQActive]);
       const unsigned long long wq = ldru64(&D.ctrl->wq[kQPull]);
-      if (np + nq == 0 && wp == 0 && wq == 0) break;
+      if (np + nq == 0 && wp == 0 && wq == 0) {
+        if (fresh) break;
+        since = C.interval;
+        G.sync();
+        continue;
+      }
       if (round >= C.max_rounds) {
         conv = false;
         break;
@@ -1711,14 +1761,17 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
       ph_push<false>(D, r, D.wl, np, C.kc, PM_ONLY_B, nullptr, WP, t);
       ph_push<true>(D, r, D.wl2, nq, C.kc, PM_ONLY_A, nullptr, WQ, t);
       G.sync();
-      ph_repair(D, r, D.wl, np, RM_B, WP, t);
-      ph_repair(D, r, D.wl2, nq, RM_PULL_A, WQ, t);
-      G.sync();
-      if (is_block0_thread0()) {  // next append follows the relabel's barriers
+      // every block read the list counters before the push barrier; the next
+      // append comes after the repair barrier
+      if (is_block0_thread0()) {
         D.ctrl->wl_cnt = D.ctrl->wl2_cnt = 0;
         D.ctrl->wq[kQActive] = D.ctrl->wq[kQPull] = 0;
       }
+      ph_repair(D, r, D.wl, np, RM_B, WP, t);
+      ph_repair(D, r, D.wl2, nq, RM_PULL_A, WQ, t);
+      G.sync();
       ++round;
+      ++since;
     }
     // ---- part_final (code/dyn_pp_bench.cu:91-152)
     if (conv) {

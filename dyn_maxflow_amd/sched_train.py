@@ -73,8 +73,6 @@ def collect(out_dir: str, quick: bool = False) -> None:
             base = dmf.default_params(g)
             for variant in VARIANTS:
                 for interval in ([1, 3] if quick else INTERVALS):
-                    if variant == "pp" and interval > 1:
-                        continue  # the push-pull batch loop relabels every round
                     eng.reset()
                     eng.static_solve(base)
                     p = dmf.default_params(g, variant=variant, relabel_interval=interval)

@@ -96,6 +96,14 @@ int dmf_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double 
                  uint64_t seed, int32_t cap_lo, int32_t cap_hi, int64_t* k,
                  int32_t** u, int32_t** v, int32_t** w);
 
+/* R-MAT over n vertices (any n >= 2): `samples` draws on the enclosing power
+ * of two; draws with an endpoint >= n are dropped like self-loops and
+ * duplicates.  For n = 2^scale and samples = n * edge_factor the output equals
+ * dmf_gen_rmat's.  Used for the uneven independent instances of the batcher. */
+int dmf_gen_rmat_n(int32_t n, int64_t samples, double a, double b, double c, uint64_t seed,
+                   int32_t cap_lo, int32_t cap_hi, int64_t* k, int32_t** u, int32_t** v,
+                   int32_t** w);
+
 /* rows x cols 4-neighbour grid, id = r*cols + c, every neighbour pair in both
  * directions with independent capacities uniform in [cap_lo, cap_hi]. */
 int dmf_gen_grid(int32_t rows, int32_t cols, uint64_t seed, int32_t cap_lo,
@@ -133,7 +141,9 @@ typedef struct dmf_params {
   int32_t relabel_interval; /* rounds between global relabels (1 = reference)   */
   int32_t scheduler;        /* DMF_SCHED_*                                       */
   int32_t max_rounds;       /* safety cap on rounds per solve (0 = default)     */
-  int32_t mode;             /* 0 = persistent one-launch solve, 1 = stepped     */
+  int32_t mode;             /* 0 = persistent one-launch solve, 1 = stepped,
+                               2 = stepped with the BFS levels replayed from
+                               captured hipGraph segments                     */
 } dmf_params;
 
 /* Fills the reference's fixed schedule: data variant, kc = max(1, m/n),

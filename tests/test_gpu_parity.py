@@ -17,7 +17,7 @@ from util import caps_after, rmat_case
 pytestmark = pytest.mark.gpu
 
 VARIANTS = ["data", "topo", "pp"]
-MODES = [0, 1]  # 0 persistent, 1 stepped
+MODES = [0, 1, 2]  # 0 persistent, 1 stepped, 2 stepped with graph-replayed BFS levels
 
 
 def say(*a):
@@ -167,12 +167,14 @@ def test_disconnected_source_and_sink(gpu):
                 assert eng.min_cut().tolist() == [1, 1, 0, 0, 0, 0]
 
 
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("interval", [2, 5])
-def test_relabel_interval_keeps_results(gpu, interval):
+def test_relabel_interval_keeps_results(gpu, interval, variant, mode):
     dmf = gpu
     g, s, t, (u, v, w) = rmat_case(12, 16, 6)
-    p = dmf.default_params(g, relabel_interval=interval)
-    say(f"[interval] {interval}")
+    p = dmf.default_params(g, variant=variant, relabel_interval=interval, mode=mode)
+    say(f"[interval] {interval} variant={variant} mode={mode}")
     with dmf.Engine(g, s, t) as eng:
         f = eng.static_solve(p)
         check_engine(eng, g, g.cap, s, t, f, "static")

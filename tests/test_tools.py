@@ -96,3 +96,19 @@ def test_driver_argument_errors(name, tmp_path):
     assert r.returncode != 0 and "i/o error" in r.stderr
     r = subprocess.run([exe, *args, "--bogus"], capture_output=True, text=True)
     assert r.returncode == 1 and "unknown flag" in r.stderr
+
+
+def test_rmat_n_matches_power_of_two_and_stays_in_range():
+    u, v, w = dmf.gen_rmat(10, 8, 5)
+    u2, v2, w2 = dmf.gen_rmat_n(1024, 1024 * 8, 5)
+    assert np.array_equal(u, u2) and np.array_equal(v, v2) and np.array_equal(w, w2)
+    for n in (1025, 1500, 3000, 40000):
+        u, v, w = dmf.gen_rmat_n(n, n * 12, 7)
+        assert len(u) > n  # enough arcs survive the range filter
+        assert u.max() < n and v.max() < n and u.min() >= 0 and v.min() >= 0
+        assert not np.any(u == v)
+        keys = u.astype(np.int64) * n + v
+        assert len(np.unique(keys)) == len(keys)
+        assert w.min() >= 1 and w.max() <= 100
+    with pytest.raises(dmf.DmfError):
+        dmf.gen_rmat_n(1, 10, 1)

@@ -188,6 +188,8 @@ constexpr int kNoGraph = 1;  // internal: capture refused, use per-level launche
 // Captures a segment of 2^idx (level, advance) pairs for BFS mode `mode`.
 int capture_segment(dmf_engine* E, int mode, int idx) {
   if (E->seg_exec[mode][idx]) return DMF_OK;
+  // fixed at capture: at most 8 blocks per CU (a full grid makes every empty
+  // trailing level of a segment expensive)
   const int grid = std::min(phase_blocks(E->n, chunks_of(E->D.wide_packed)), std::max(1, E->num_cu) * 8);
   hipStream_t st = E->stream;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
@@ -201,12 +203,12 @@ int capture_segment(dmf_engine* E, int mode, int idx) {
   hipGraph_t g = nullptr;
   const bool ended = hipStreamEndCapture(st, &g) == hipSuccess;
   if (!ok || !ended || !g) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return kNoGraph;
   }
   hipGraphExec_t x = nullptr;
   if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     return kNoGraph;
   }
   E->seg_graph[mode][idx] = g;
@@ -217,8 +219,8 @@ int capture_segment(dmf_engine* E, int mode, int idx) {
 void destroy_segments(dmf_engine* E) {
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < dmf_engine::kSegSizes; ++b) {
-      if (E->seg_exec[a][b]) hipGraphExecDestroy(E->seg_exec[a][b]);
-      if (E->seg_graph[a][b]) hipGraphDestroy(E->seg_graph[a][b]);
+      if (E->seg_exec[a][b]) (void)hipGraphExecDestroy(E->seg_exec[a][b]);
+      if (E->seg_graph[a][b]) (void)hipGraphDestroy(E->seg_graph[a][b]);
       E->seg_exec[a][b] = nullptr;
       E->seg_graph[a][b] = nullptr;
     }

@@ -500,7 +500,11 @@ __device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int 
   if (mode == BFS_SINK) return D.rres[e] > 0;
   if (mode == BFS_PP) {
     if (is_term(D, w)) return false;
-    return vB ? (D.rres[e] > 0) : (D.res[e] > 0);
+    // both residuals are read and the value selected: selecting between the
+    // two array fields per lane would force the by-value kernel argument into
+    // per-lane scratch memory
+    const int32_t back = D.rres[e], fwd = D.res[e];
+    return (vB ? back : fwd) > 0;
   }
   // BFS_FINAL: non-B neighbours of the stranded part only
   if (D.side[w] || !D.pfinal[w]) return false;

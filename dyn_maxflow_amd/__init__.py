@@ -32,6 +32,7 @@ EDEVICE = -5
 ENOCONV = -6
 EFORMAT = -7
 EADMIT = -8
+EAGAIN = -9
 
 VARIANT_DATA = 0
 VARIANT_TOPO = 1
@@ -91,6 +92,60 @@ class Counters(C.Structure):
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    """dmf_server_config."""
+
+    _fields_ = [
+        ("hbm_budget", C.c_int64),
+        ("staging_bytes", C.c_int64),
+        ("max_sessions", C.c_int32),
+        ("pad", C.c_int32),
+        ("params", Params),
+    ]
+
+
+class Result(C.Structure):
+    """dmf_result: one served batch."""
+
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("session", C.c_int32),
+        ("status", C.c_int32),
+        ("flow", C.c_int64),
+        ("k", C.c_int32),
+        ("pad", C.c_int32),
+        ("submit_ms", C.c_double),
+        ("admit_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("done_ms", C.c_double),
+        ("device_ms", C.c_double),
+        ("arc_visits", C.c_uint64),
+        ("rounds", C.c_uint64),
+        ("global_relabels", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("sessions", C.c_int32),
+        ("in_flight", C.c_int32),
+        ("queued", C.c_int32),
+        ("pad", C.c_int32),
+        ("completed", C.c_uint64),
+        ("hbm_budget", C.c_int64),
+        ("hbm_held", C.c_int64),
+        ("staged_bytes", C.c_int64),
+        ("peak_staged_bytes", C.c_int64),
+        ("clock_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
 _LIB = None
@@ -157,6 +212,18 @@ def _lib():
         "dmf_batch_static_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), _I64P]),
         "dmf_batch_apply": (C.c_int, [C.c_void_p, _I32P, _I32P, _I32P, _I32P, C.POINTER(Params), _I64P]),
         "dmf_batch_flows": (C.c_int, [C.c_void_p, _I64P]),
+        "dmf_engine_footprint": (C.c_int64, [C.c_int32, C.c_int32]),
+        "dmf_engine_reserve_updates": (C.c_int, [C.c_void_p, C.c_int32]),
+        "dmf_server_config_default": (None, [C.POINTER(ServerConfig)]),
+        "dmf_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+        "dmf_server_open": (C.c_int, [C.c_void_p, C.POINTER(_Csr), C.c_int32, C.c_int32, C.c_int32,
+                                      _I32P, _I64P]),
+        "dmf_server_submit": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _I32P, _I32P, _I32P,
+                                        C.POINTER(C.c_uint64)]),
+        "dmf_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, C.POINTER(Result)]),
+        "dmf_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+        "dmf_server_close": (C.c_int, [C.c_void_p, C.c_int32]),
+        "dmf_server_destroy": (C.c_int, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -412,11 +479,97 @@ class Engine:
     def set_stream(self, stream_ptr: int) -> None:
         _check(_lib().dmf_engine_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
 
+    def reserve_updates(self, k: int) -> None:
+        _check(_lib().dmf_engine_reserve_updates(self._h, k), "reserve_updates")
+
     def autotune(self, params: Params):
         """Picks the fastest (lane_max, wide_t) degree-bucket thresholds; resets the engine."""
         a, b = C.c_int32(), C.c_int32()
         _check(_lib().dmf_engine_autotune(self._h, C.byref(params), C.byref(a), C.byref(b)), "autotune")
         return a.value, b.value
+
+
+def engine_footprint(n: int, m: int) -> int:
+    return int(_lib().dmf_engine_footprint(n, m))
+
+
+def server_config(hbm_budget=0, staging_bytes=0, max_sessions=0, params: Params | None = None,
+                  **param_fields) -> ServerConfig:
+    c = ServerConfig()
+    _lib().dmf_server_config_default(C.byref(c))
+    c.hbm_budget = hbm_budget
+    c.staging_bytes = staging_bytes
+    if max_sessions:
+        c.max_sessions = max_sessions
+    if params is not None:
+        c.params = params
+    for k, v in param_fields.items():
+        setattr(c.params, k, VARIANTS[v] if k == "variant" and isinstance(v, str) else v)
+    return c
+
+
+class Server:
+    """Streaming update server (dmf_server_*): resident sessions, batches
+    admitted by device memory, solved as they arrive."""
+
+    def __init__(self, config: ServerConfig | None = None):
+        self._h = C.c_void_p()
+        self._graphs = {}
+        _check(_lib().dmf_server_create(C.byref(config) if config is not None else None, C.byref(self._h)),
+               "server_create")
+
+    def open(self, g: Graph, s: int, t: int, max_batch: int):
+        """Returns (session id, static flow)."""
+        sid = C.c_int32()
+        f = C.c_int64()
+        c = g._csr()
+        _check(_lib().dmf_server_open(self._h, C.byref(c), s, t, max_batch, C.byref(sid), C.byref(f)),
+               "server_open")
+        self._graphs[sid.value] = g
+        return sid.value, f.value
+
+    def submit(self, session: int, src, dst, w) -> int:
+        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        tk = C.c_uint64()
+        _check(_lib().dmf_server_submit(self._h, session, len(src), _ptr(src), _ptr(dst), _ptr(w),
+                                        C.byref(tk)), "server_submit")
+        return tk.value
+
+    def submit_status(self, session: int, src, dst, w) -> int:
+        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        tk = C.c_uint64()
+        return _lib().dmf_server_submit(self._h, session, len(src), _ptr(src), _ptr(dst), _ptr(w), C.byref(tk))
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0) -> Result:
+        r = Result()
+        _check(_lib().dmf_server_wait(self._h, ticket, timeout_ms, C.byref(r)), "server_wait")
+        return r
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib().dmf_server_get_stats(self._h, C.byref(st)), "server_stats")
+        return st.as_dict()
+
+    def close(self, session: int) -> None:
+        _check(_lib().dmf_server_close(self._h, session), "server_close")
+        self._graphs.pop(session, None)
+
+    def destroy(self) -> None:
+        if self._h:
+            _lib().dmf_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class BatchEngine(Engine):

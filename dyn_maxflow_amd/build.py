@@ -5,6 +5,7 @@ Produces, next to this file:
   bin/dyn_{data,pp,topo}_bench{,_large}   drivers with the reference's argv/stdout
   bin/dmf_input             seeded input tool (graph / update .bin writer)
   bin/dmf_multi_bench       many independent instances solved as one batched engine
+  bin/dmf_serve_bench       streaming update batches against a resident graph (server)
 
 Everything is compiled for gfx950 with hipcc; packed-fp32 instruction
 selection is disabled for all device code.
@@ -63,6 +64,12 @@ def build(verbose: bool = False) -> str:
     host_obj = os.path.join(BUILD, "host_io.o")
     _run(["g++", *HOST_FLAGS, "-I", INCLUDE, "-c", host_src, "-o", host_obj])
     objs.append(host_obj)
+    srv_src = os.path.join(CSRC, "server.cpp")
+    if os.path.exists(srv_src):
+        srv_obj = os.path.join(BUILD, "server.o")
+        _run(["g++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-I", INCLUDE,
+              "-c", srv_src, "-o", srv_obj])
+        objs.append(srv_obj)
     eng_src = os.path.join(CSRC, "engine.hip")
     eng_obj = os.path.join(BUILD, "engine.o")
     _run([HIPCC, *DEVICE_FLAGS, "-I", INCLUDE, "-c", eng_src, "-o", eng_obj])
@@ -83,10 +90,11 @@ def build(verbose: bool = False) -> str:
             _run(["g++", *HOST_FLAGS, "-I", INCLUDE, f"-DDMF_BENCH_VARIANT=\"{variant}\"",
                   f"-DDMF_BENCH_LARGE={large}", drv, "-o", os.path.join(BIN, name),
                   "-L", HERE, "-ldmf", f"-Wl,-rpath,$ORIGIN/.."])
-    multi = os.path.join(CSRC, "multi_bench.cpp")
-    if os.path.exists(multi):
-        _run(["g++", *HOST_FLAGS, "-I", INCLUDE, multi, "-o", os.path.join(BIN, "dmf_multi_bench"),
-              "-L", HERE, "-ldmf", "-pthread", f"-Wl,-rpath,$ORIGIN/.."])
+    for src_name, exe in (("multi_bench.cpp", "dmf_multi_bench"), ("serve_bench.cpp", "dmf_serve_bench")):
+        src = os.path.join(CSRC, src_name)
+        if os.path.exists(src):
+            _run(["g++", *HOST_FLAGS, "-I", INCLUDE, src, "-o", os.path.join(BIN, exe),
+                  "-L", HERE, "-ldmf", "-pthread", f"-Wl,-rpath,$ORIGIN/.."])
     tool = os.path.join(CSRC, "input_tool.cpp")
     if os.path.exists(tool):
         _run(["g++", *HOST_FLAGS, "-I", INCLUDE, tool, "-o", os.path.join(BIN, "dmf_input"),

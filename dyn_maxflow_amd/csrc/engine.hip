@@ -148,6 +148,13 @@ int phase_blocks(int64_t cnt, int64_t chunks) {
 
 int64_t chunks_of(unsigned long long packed) { return static_cast<int64_t>(packed & 0xffffffffu); }
 
+// BFS levels: a wave expands kGroup frontier vertices at a time, so one wave
+// per kGroup vertices (plus one per wide chunk) puts every group in flight.
+int bfs_blocks(int64_t cnt, int64_t chunks) {
+  const int64_t waves = (cnt + kGroup - 1) / kGroup + chunks;
+  return blocks_for(waves * kWave);
+}
+
 // Stepped global relabel: host loop over levels.  Returns visited count.
 int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
   Dev& D = E->D;
@@ -162,15 +169,10 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
     const int32_t cnt = E->h_ctrl->qcnt[lvl % 3];
     const unsigned long long wpk = E->h_ctrl->wq[lvl % 3];
     if (cnt == 0) break;
-    if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) return rc;
-    if ((rc = zero_dev(E, &D.ctrl->wq[(lvl + 1) % 3], sizeof(unsigned long long)))) return rc;
-    if (use_bottom_up(D, mode, cnt)) {
+    if (use_bottom_up(D, mode, cnt, chunks_of(wpk))) {
       LAUNCH(k_bfs_level, blocks_for(E->n), D, mode, lo, cnt, lvl, wpk, 1);
-    } else if (use_collect(D, cnt)) {
-      LAUNCH(k_bfs_level, phase_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 2);
-      LAUNCH(k_bfs_collect, blocks_for(E->n), D, lo, cnt, lvl);
     } else {
-      LAUNCH(k_bfs_level, phase_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 0);
+      LAUNCH(k_bfs_level, bfs_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 0);
     }
     lo += cnt;
     ++lvl;
@@ -264,7 +266,7 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
 
 int relabel(dmf_engine* E, int mode, int part_cnt, int* visited) {
   const Dev& D = E->D;
-  if (E->use_graph && !E->graph_failed && D.bottom_up == 0 && D.collect == 0) {
+  if (E->use_graph && !E->graph_failed) {
     const int rc = relabel_graph(E, mode, part_cnt, visited);
     if (rc != kNoGraph) return rc;
     // capture or instantiation refused: keep the host-driven level loop for
@@ -724,10 +726,11 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
   D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
   {
+    // direction-optimizing relabel: off unless DMF_BFS_BOTTOM_UP gives the
+    // divisor (on residual graphs near a maximum flow the source side is
+    // unreachable and a bottom-up level scans all of it)
     const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
     D.bottom_up = bu ? std::atoi(bu) : 0;
-    const char* co = std::getenv("DMF_BFS_COLLECT");
-    D.collect = co ? std::atoi(co) : 0;
   }
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
     dmf_engine_destroy(E);
@@ -1037,6 +1040,27 @@ int dmf_engine_counters(dmf_engine* E, dmf_counters* c) {
 }
 
 int64_t dmf_engine_device_bytes(const dmf_engine* E) { return E ? E->bytes : 0; }
+
+int64_t dmf_engine_footprint(int32_t n, int32_t m) {
+  if (n < 0 || m < 0) return 0;
+  const int64_t N = n, M = m;
+  uint64_t hcap = 16;
+  while (hcap < 2 * static_cast<uint64_t>(M) + 16) hcap <<= 1;
+  int64_t b = 4 * (N + 1);                      // row
+  b += 7 * 4 * M;                               // col rev cap cap0 res rres owner
+  b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst
+  b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
+  b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
+  b += 48 * (M / 64 + 1);                       // wide lists and queues (wide_t >= 64)
+  b += static_cast<int64_t>(C_N) * kSlots * 8;  // counters
+  b += int64_t{1} << 20;                        // control block, scheduler, allocation slack
+  return b;
+}
+
+int dmf_engine_reserve_updates(dmf_engine* E, int32_t k) {
+  if (!E || k < 0) return DMF_EINVAL;
+  return ensure_update_capacity(E, std::max(k, 1));
+}
 
 int32_t dmf_engine_instances(const dmf_engine* E) { return E ? E->ninst : 0; }
 

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -135,6 +136,7 @@ const char* dmf_strerror(int status) {
     case DMF_ENOCONV: return "round cap reached before convergence";
     case DMF_EFORMAT: return "malformed file";
     case DMF_EADMIT: return "request exceeds the admission budget";
+    case DMF_EAGAIN: return "result not ready";
     default: return "unknown status";
   }
 }
@@ -504,13 +506,20 @@ int dmf_gen_updates(int64_t k, const int32_t* u, const int32_t* v, int32_t* cur_
   }
   Xoshiro256ss rng(seed);
   // Partial Fisher-Yates over edge indices: distinct edges, no replacement.
-  std::vector<int64_t> perm(static_cast<size_t>(k));
-  for (int64_t i = 0; i < k; ++i) perm[i] = i;
+  // The permutation is kept sparse (positions displaced so far; every other
+  // position still holds its own index), so a batch costs O(count), not O(k).
+  std::unordered_map<int64_t, int64_t> moved;
+  moved.reserve(static_cast<size_t>(std::min<int64_t>(k, 2 * count + 16)));
+  auto at = [&moved](int64_t x) {
+    const auto it = moved.find(x);
+    return it == moved.end() ? x : it->second;
+  };
   int64_t got = 0;
   for (int64_t i = 0; i < k && got < count; ++i) {
     const int64_t j = i + static_cast<int64_t>(rng.below(static_cast<uint64_t>(k - i)));
-    std::swap(perm[i], perm[j]);
-    const int64_t e = perm[i];
+    const int64_t e = at(j);
+    if (j != i) moved[j] = at(i);  // position i is never read again
+    moved.erase(i);
     if (u[e] == v[e]) continue;
     const bool inc = rng.uniform() < inc_frac;
     const int32_t cur = cur_w[e];

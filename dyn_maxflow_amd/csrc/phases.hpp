@@ -109,8 +109,7 @@ struct Dev {
   uint8_t* pfinal;
   Ctrl* ctrl;
   unsigned long long* ctrs;  // C_N x kSlots work counters
-  int32_t bottom_up;         // frontier size divisor for bottom-up BFS levels (0 = never)
-  int32_t collect;           // frontier size divisor for plain-store + collect levels (0 = never)
+  int32_t bottom_up;         // bottom-up BFS level once frontier arcs > m / bottom_up (0 = never)
   unsigned long long* hkey;
   int32_t* hval;
   unsigned long long hmask;
@@ -281,6 +280,8 @@ __device__ __forceinline__ int wave_incl_scan_i32(int x) {
   }
   return x;
 }
+
+__device__ __forceinline__ bool is_block0_thread0() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
 struct Range {
   int gid;  // global thread index
@@ -519,21 +520,28 @@ __device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int
   if (hit && is_wide(D, w)) wq_append(D, qnext, w, degree(D, w));
 }
 
+// A vertex is unlabelled while its height is n; a level claims it by
+// compare-and-swap from n (one winner per vertex).  Claims stay on distinct
+// words: a packed one-bit-per-vertex mark array was measured slower, its
+// atomics colliding on the words 32 vertices share.
+__device__ __forceinline__ bool bfs_unlabelled(const Dev& D, int w) { return D.height[w] == D.n; }
+__device__ __forceinline__ bool bfs_claim(const Dev& D, int w, int nb) {
+  return atomicCAS(&D.height[w], D.n, nb) == D.n;
+}
+
 // Top-down BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
 // code/dyn_pp.h:99-120; final_bfs, code/dyn_pp.h:123-140).  Frontier is
 // order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
-// get height lvl+1 (one winner per vertex through compare-and-swap) and are
-// appended after the frontier through per-wave LDS staging.  The 64 frontier
-// vertices a wave holds are expanded as one flattened arc range, so
-// consecutive lanes read consecutive arcs (coalesced col/residual loads) and
-// each lane finds its arc's tail by a 6-step binary search over the wave's
-// degree prefix sums.
+// get height lvl+1 (one winner per vertex) and are appended after the frontier
+// through per-wave LDS staging.  The 16 narrow frontier vertices a wave takes
+// at a time are expanded as one flattened arc range, so consecutive lanes read
+// consecutive arcs (coalesced col/residual loads) and each lane finds its
+// arc's tail by a 4-step binary search over the group's degree prefix sums.
+// Both loops keep two 64-arc tiles in flight per wave.
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
-                             int32_t* next_cnt, const WList& W, int qnext, Tally& t,
-                             bool plain = false) {
+                             int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
   Stage st{sbuf[threadIdx.x / kWave], 0};
-  const int n = D.n;
   const int nb = lvl + 1;
   int32_t* out = D.order + lo + cnt;
   // wide frontier vertices: chunks spread over every wave of the grid
@@ -541,26 +549,23 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     int a, b;
     const int v = chunk_range(D, W, c, &a, &b);
     const bool vB = mode == BFS_PP && D.side[v] != 0;
-    for (int j0 = a; j0 < b; j0 += kWave) {
-      const int e = j0 + lane_id();
-      bool hit = false;
-      int w = -1;
-      if (e < b) {
-        w = D.col[e];
-        t.arc += 1;
-        if (bfs_arc_ok(D, mode, vB, e, w) && D.height[w] == n) {
-          if (plain) {
-            D.height[w] = nb;  // same value from every writer; collected by ph_bfs_collect
-          } else {
-            hit = atomicCAS(&D.height[w], n, nb) == n;
-            t.atom += 1;
-          }
-        }
-      }
-      if (!plain) {
-        bfs_note_wide(D, hit, w, qnext);
-        st.push(hit, w, next_cnt, out);
-      }
+    for (int j0 = a; j0 < b; j0 += 2 * kWave) {
+      const int eA = j0 + lane_id(), eB = eA + kWave;
+      const bool inA = eA < b, inB = eB < b;
+      const int wA = inA ? D.col[eA] : 0;
+      const int wB = inB ? D.col[eB] : 0;
+      bool cA = inA && bfs_arc_ok(D, mode, vB, eA, wA);
+      bool cB = inB && bfs_arc_ok(D, mode, vB, eB, wB);
+      cA = cA && bfs_unlabelled(D, wA);
+      cB = cB && bfs_unlabelled(D, wB);
+      t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
+      t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
+      const bool hitA = cA && bfs_claim(D, wA, nb);
+      const bool hitB = cB && bfs_claim(D, wB, nb);
+      bfs_note_wide(D, hitA, wA, qnext);
+      bfs_note_wide(D, hitB, wB, qnext);
+      st.push(hitA, wA, next_cnt, out);
+      st.push(hitB, wB, next_cnt, out);
     }
   }
   for (int base = r.wid * kGroup; base < cnt; base += r.nw * kGroup) {
@@ -578,7 +583,6 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     const int incl = wave_incl_scan_i32(deg);
     const int total = __shfl(incl, kWave - 1);
     const int excl = incl - deg;
-    // two 64-arc tiles per iteration keep twice the loads in flight
     for (int k0 = 0; k0 < total; k0 += 2 * kWave) {
       const int pA = k0 + lane_id(), pB = pA + kWave;
       int posA = 0, posB = 0;  // owner lanes: number of group lanes whose prefix end <= p
@@ -593,51 +597,21 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
       const int eB = __shfl(b, posB) + (pB - __shfl(excl, posB));
       const bool okA = __shfl(vB, posA) != 0, okB = __shfl(vB, posB) != 0;
       const bool inA = pA < total, inB = pB < total;
-      int wA = -1, wB = -1;
-      bool cA = false, cB = false;
-      if (inA) {
-        wA = D.col[eA];
-        cA = bfs_arc_ok(D, mode, okA, eA, wA);
-      }
-      if (inB) {
-        wB = D.col[eB];
-        cB = bfs_arc_ok(D, mode, okB, eB, wB);
-      }
-      cA = cA && D.height[wA] == n;
-      cB = cB && D.height[wB] == n;
+      const int wA = inA ? D.col[eA] : 0;
+      const int wB = inB ? D.col[eB] : 0;
+      bool cA = inA && bfs_arc_ok(D, mode, okA, eA, wA);
+      bool cB = inB && bfs_arc_ok(D, mode, okB, eB, wB);
+      cA = cA && bfs_unlabelled(D, wA);
+      cB = cB && bfs_unlabelled(D, wB);
       t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
-      bool hitA = false, hitB = false;
-      if (plain) {
-        if (cA) D.height[wA] = nb;  // same value from every writer; collected by ph_bfs_collect
-        if (cB) D.height[wB] = nb;
-      } else {
-        if (cA) hitA = atomicCAS(&D.height[wA], n, nb) == n;
-        if (cB) hitB = atomicCAS(&D.height[wB], n, nb) == n;
-        t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
-        bfs_note_wide(D, hitA, wA, qnext);
-        bfs_note_wide(D, hitB, wB, qnext);
-        st.push(hitA, wA, next_cnt, out);
-        st.push(hitB, wB, next_cnt, out);
-      }
+      t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
+      const bool hitA = cA && bfs_claim(D, wA, nb);
+      const bool hitB = cB && bfs_claim(D, wB, nb);
+      bfs_note_wide(D, hitA, wA, qnext);
+      bfs_note_wide(D, hitB, wB, qnext);
+      st.push(hitA, wA, next_cnt, out);
+      st.push(hitB, wB, next_cnt, out);
     }
-  }
-  st.flush(next_cnt, out);
-}
-
-// Frontier collection after a plain-store level: every vertex whose height is
-// now lvl+1 is appended (once, by its own lane) to order[] after the old
-// frontier, and wide ones to wide queue qnext.  One pass over height[n].
-__device__ void ph_bfs_collect(const Dev& D, const Range& r, int lvl, int32_t* next_cnt, int qnext,
-                               int32_t* out, Tally& t) {
-  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
-  Stage st{sbuf[threadIdx.x / kWave], 0};
-  const int nb = lvl + 1;
-  for (int base = r.wid * kWave; base < D.n; base += r.nw * kWave) {
-    const int v = base + lane_id();
-    const bool hit = v < D.n && D.height[v] == nb;
-    t.vert += 1;
-    bfs_note_wide(D, hit, v, qnext);
-    st.push(hit, v, next_cnt, out);
   }
   st.flush(next_cnt, out);
 }
@@ -653,17 +627,22 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
   Stage st{sbuf[threadIdx.x / kWave], 0};
   const int n = D.n;
+  const int32_t* __restrict__ col = D.col;
+  const int32_t* __restrict__ res = D.res;
+  const int32_t* height = D.height;
   for (int base = r.wid * kWave; base < n; base += r.nw * kWave) {
     const int w = base + lane_id();
     bool cand = false;
     int b = 0, deg = 0;
-    if (w < n && D.height[w] == n) {
+    if (w < n && height[w] == n) {
       b = D.row[w];
       deg = D.row[w + 1] - b;
       cand = deg > 0;
       t.vert += 1;
     }
     bool hit = false;
+    // candidates above lane_max: the whole wave scans, two 64-arc tiles per
+    // step, and stops at the first tile holding a parent
     unsigned long long big = __ballot(cand && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
@@ -671,27 +650,38 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
       const int bb = __shfl(b, leader);
       const int bd = __shfl(deg, leader);
       bool found = false;
-      for (int j0 = 0; j0 < bd && !found; j0 += kWave) {
-        const int e = bb + j0 + lane_id();
-        bool f = false;
-        if (j0 + lane_id() < bd) {
-          t.arc += 1;
-          f = D.res[e] > 0 && D.height[D.col[e]] == lvl;
-        }
-        found = __ballot(f) != 0;
+      for (int j0 = 0; j0 < bd && !found; j0 += 2 * kWave) {
+        const int jA = j0 + lane_id(), jB = jA + kWave;
+        const bool inA = jA < bd, inB = jB < bd;
+        const int xA = inA ? col[bb + jA] : 0, rA = inA ? res[bb + jA] : 0;
+        const int xB = inB ? col[bb + jB] : 0, rB = inB ? res[bb + jB] : 0;
+        const int hA = rA > 0 ? height[xA] : -1;
+        const int hB = rB > 0 ? height[xB] : -1;
+        t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
+        found = __ballot(hA == lvl || hB == lvl) != 0;
       }
       if (lane_id() == leader) hit = found;
     }
+    // narrow candidates: each lane scans its own arcs, four independent
+    // (col, res) pairs and height gathers in flight per step
     if (cand && deg <= D.lane_max) {
-      for (int e = b; e < b + deg; ++e) {
-        t.arc += 1;
-        if (D.res[e] > 0 && D.height[D.col[e]] == lvl) {
-          hit = true;
-          break;
-        }
+      const int end = b + deg;
+      for (int e = b; e < end && !hit; e += 4) {
+        const bool i1 = e + 1 < end, i2 = e + 2 < end, i3 = e + 3 < end;
+        const int x0 = col[e], r0 = res[e];
+        const int x1 = i1 ? col[e + 1] : 0, r1 = i1 ? res[e + 1] : 0;
+        const int x2 = i2 ? col[e + 2] : 0, r2 = i2 ? res[e + 2] : 0;
+        const int x3 = i3 ? col[e + 3] : 0, r3 = i3 ? res[e + 3] : 0;
+        const int h0 = r0 > 0 ? height[x0] : -1;
+        const int h1 = r1 > 0 ? height[x1] : -1;
+        const int h2 = r2 > 0 ? height[x2] : -1;
+        const int h3 = r3 > 0 ? height[x3] : -1;
+        t.arc += 1 + (i1 ? 1 : 0) + (i2 ? 1 : 0) + (i3 ? 1 : 0);
+        hit = h0 == lvl || h1 == lvl || h2 == lvl || h3 == lvl;
       }
     }
     if (hit) {
+      // w is owned by this thread alone: a plain store labels it
       D.height[w] = lvl + 1;
       if (deg > D.wide_t) wq_append(D, qnext, w, deg);
     }
@@ -699,6 +689,20 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
   }
   st.flush(next_cnt, out);
 }
+
+// Direction choice for one level of the relabel from t, after Beamer, Asanovic
+// and Patterson, "Direction-optimizing breadth-first search" (SC 2012):
+// bottom-up while the frontier's arcs exceed m / D.bottom_up.  The frontier's
+// arcs are estimated as its narrow vertices times the mean degree plus its
+// wide chunks times kChunk.  R-MAT frontiers are small in vertices but hold
+// the hubs' arcs, so the estimate counts arcs, not vertices.
+__host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, int cnt, long long chunks) {
+  if (mode != BFS_SINK || D.bottom_up <= 0) return false;
+  const long long mean = D.n > 0 ? D.m / D.n : 0;
+  const long long arcs = static_cast<long long>(cnt) * mean + chunks * kChunk;
+  return arcs * D.bottom_up > D.m;
+}
+
 
 // ---------------------------------------------------------------- worklists
 // create_wl (code/dyn_data.h:217-238) over the labelled vertices order[0,cnt):
@@ -1418,14 +1422,18 @@ __global__ void __launch_bounds__(kBlock) k_bfs_init_final(Dev D, int cnt) {
   ph_bfs_init_final(D, make_range(), cnt, t);
   t.flush(D.ctrs);
 }
+// One level of the stepped relabel.  Block 0 also clears ring slot
+// (lvl+2)%3: it held the size of level lvl-1, which the host has read, and
+// level lvl+1 appends to it.
 __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, int cnt, int lvl,
                                                       unsigned long long wpk, int bottomup) {
   Tally t;
   const Range r = make_range();
-  if (bottomup == 2) {
-    const WList W = wlist(D, lvl % 3, wpk);
-    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, true);
-  } else if (bottomup) {
+  if (is_block0_thread0()) {
+    D.ctrl->qcnt[(lvl + 2) % 3] = 0;
+    D.ctrl->wq[(lvl + 2) % 3] = 0;
+  }
+  if (bottomup) {
     ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
   } else {
     const WList W = wlist(D, lvl % 3, wpk);
@@ -1449,8 +1457,13 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
   if (cnt == 0) return;  // uniform: the BFS is over
   const int lo = ldr(&D.ctrl->blo);
   Tally t;
-  const WList W = wlist(D, lvl % 3, ldru64(&D.ctrl->wq[lvl % 3]));
-  ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+  const unsigned long long wpk = ldru64(&D.ctrl->wq[lvl % 3]);
+  if (use_bottom_up(D, mode, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
+    ph_bfs_bottomup(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+  } else {
+    const WList W = wlist(D, lvl % 3, wpk);
+    ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+  }
   t.flush(D.ctrs);
 }
 // One thread: moves the cursor past the level just expanded.  Slot (lvl+2)%3
@@ -1466,11 +1479,6 @@ __global__ void k_bfs_advance(const Dev* __restrict__ pD) {
   c->wq[(lvl + 2) % 3] = 0;
   c->blo += cnt;
   c->blvl = lvl + 1;
-}
-__global__ void __launch_bounds__(kBlock) k_bfs_collect(Dev D, int lo, int cnt, int lvl) {
-  Tally t;
-  ph_bfs_collect(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
-  t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
   Tally t;
@@ -1569,21 +1577,6 @@ __global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
 // takes the same branch.  A counter is cleared only after a barrier that every
 // reader of it has passed, and before the barrier preceding its next writer.
 
-__device__ __forceinline__ bool is_block0_thread0() { return blockIdx.x == 0 && threadIdx.x == 0; }
-
-// Direction choice for one level of the relabel from t: bottom-up once the
-// frontier holds more than n / D.bottom_up vertices.
-__host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, int cnt) {
-  return mode == BFS_SINK && D.bottom_up > 0 && static_cast<long long>(cnt) * D.bottom_up > D.n;
-}
-
-// Plain-store expansion plus a collection pass over height[] once the frontier
-// holds more than n / D.collect vertices: no compare-and-swap or append atomic
-// per labelled vertex, at the price of one scan of n heights.
-__host__ __device__ __forceinline__ bool use_collect(const Dev& D, int cnt) {
-  return D.collect > 0 && static_cast<long long>(cnt) * D.collect > D.n;
-}
-
 // Global relabel with the given root mode; returns the number of vertices
 // labelled (order[0, visited)).  On entry ctrl->qcnt[] and ctrl->wq[0..2] are
 // 0; on exit they are 0 again.
@@ -1597,13 +1590,8 @@ __device__ int relabel_persistent(const Dev& D, cg::grid_group& G, const Range& 
     const int cur = lvl % 3;
     const int cnt = ldr(&D.ctrl->qcnt[cur]);
     if (cnt == 0) break;
-    if (use_bottom_up(D, mode, cnt)) {
+    if (use_bottom_up(D, mode, cnt, static_cast<long long>(ldru64(&D.ctrl->wq[cur]) & 0xffffffffu))) {
       ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
-    } else if (use_collect(D, cnt)) {
-      const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
-      ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, true);
-      G.sync();
-      ph_bfs_collect(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
     } else {
       const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));
       ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);

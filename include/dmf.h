@@ -28,6 +28,7 @@ extern "C" {
 #define DMF_ENOCONV (-6)  /* round cap reached before convergence               */
 #define DMF_EFORMAT (-7)  /* file contents inconsistent (negative sizes, ...)   */
 #define DMF_EADMIT (-8)   /* serving: request exceeds the admission budget      */
+#define DMF_EAGAIN (-9)   /* serving: result not ready within the timeout       */
 
 /* Human-readable text of a status code (static storage). */
 const char* dmf_strerror(int status);
@@ -220,6 +221,14 @@ int dmf_engine_counters(dmf_engine* e, dmf_counters* c);
 /* Bytes of device memory held by this engine. */
 int64_t dmf_engine_device_bytes(const dmf_engine* e);
 
+/* Upper bound of dmf_engine_create's device bytes for a graph of n vertices
+ * and m arcs (update buffers excluded: 16 bytes per reserved update). */
+int64_t dmf_engine_footprint(int32_t n, int32_t m);
+
+/* Sizes the update buffers for batches of up to k records now, so that later
+ * batches of that size allocate nothing. */
+int dmf_engine_reserve_updates(dmf_engine* e, int32_t k);
+
 /* Tries a few (lane_max, wide_t) degree-bucket thresholds -- the degree up to
  * which one lane scans a vertex, and above which a vertex is split into chunks
  * spread over many waves -- with a static solve each, keeps the fastest and
@@ -242,6 +251,59 @@ int dmf_batch_static_solve(dmf_engine* e, const dmf_params* p, int64_t* flows);
 int dmf_batch_apply(dmf_engine* e, const int32_t* counts, const int32_t* src, const int32_t* dst,
                     const int32_t* w, const dmf_params* p, int64_t* flows);
 int dmf_batch_flows(dmf_engine* e, int64_t* flows);
+
+/* ----------------------------------------------------- streaming server */
+/* No reference counterpart (the reference drivers solve one update file per
+ * process).  Residual graphs stay resident; update batches are solved as they
+ * arrive.  Sessions (one resident graph each) are admitted only if their
+ * device footprint fits hbm_budget; submitted batches are admitted, in
+ * arrival order, into a device staging arena of staging_bytes when their
+ * bytes fit, else they wait until solves free space.  Each session's batches
+ * are applied in submission order by its own host thread and stream, so
+ * solves of different sessions overlap on the device and staging copies
+ * overlap with solves.  Thread-safe; results are collected once per ticket. */
+typedef struct dmf_server dmf_server;
+typedef struct dmf_server_config {
+  int64_t hbm_budget;     /* device bytes the server may hold; 0 = 90% of free */
+  int64_t staging_bytes;  /* staging arena for admitted batches; 0 = budget / 8 */
+  int32_t max_sessions;   /* 0 = 64 */
+  int32_t pad;
+  dmf_params params;      /* solve parameters of every batch (kc <= 0: per-graph default) */
+} dmf_server_config;
+typedef struct dmf_result {
+  uint64_t ticket;
+  int32_t session;
+  int32_t status;         /* DMF_OK or the batch's error */
+  int64_t flow;           /* max-flow value after this batch */
+  int32_t k;              /* update records in the batch */
+  int32_t pad;
+  double submit_ms, admit_ms, start_ms, done_ms; /* server clock */
+  double device_ms;       /* device time of the solve */
+  uint64_t arc_visits, rounds, global_relabels;
+} dmf_result;
+typedef struct dmf_server_stats {
+  int32_t sessions, in_flight, queued, pad;
+  uint64_t completed;
+  int64_t hbm_budget, hbm_held, staged_bytes, peak_staged_bytes;
+  double clock_ms;
+} dmf_server_stats;
+void dmf_server_config_default(dmf_server_config* c);  /* learned scheduler, stepped */
+int dmf_server_create(const dmf_server_config* cfg, dmf_server** out);
+/* Builds a resident session (engine + static solve); DMF_EADMIT if its
+ * footprint plus max_batch update slots does not fit the remaining budget. */
+int dmf_server_open(dmf_server* s, const dmf_csr* g, int32_t src, int32_t snk, int32_t max_batch,
+                    int32_t* session, int64_t* static_flow);
+/* Queues one batch (copied; returns at once).  DMF_EADMIT if k exceeds the
+ * session's max_batch or the batch alone exceeds the staging arena. */
+int dmf_server_submit(dmf_server* s, int32_t session, int32_t k, const int32_t* src,
+                      const int32_t* dst, const int32_t* w, uint64_t* ticket);
+/* Blocks until the ticket's result is ready (timeout_ms < 0: no limit;
+ * DMF_EAGAIN on timeout) and returns it; each ticket is returned once. */
+int dmf_server_wait(dmf_server* s, uint64_t ticket, double timeout_ms, dmf_result* out);
+int dmf_server_get_stats(dmf_server* s, dmf_server_stats* out);
+/* Finishes the session's queued batches, then frees it. */
+int dmf_server_close(dmf_server* s, int32_t session);
+int dmf_server_destroy(dmf_server* s);
 
 /* ------------------------------------------ reference in-process entry points */
 /* int recalculate_max_flow(graph&, std::vector<update>, int batchSize, int s,

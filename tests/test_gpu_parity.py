@@ -3,8 +3,9 @@
 Bar: bit-exact max-flow value and S_min (vertices reachable from s in the
 residual graph of a maximum flow) after the static solve and after every update
 batch, plus the residual-state invariants (0 <= res, pair consistency,
-excess = net inflow).  Runs every variant (data, topo, pp) under both drivers
-(persistent one-launch solve and stepped per-phase launches)."""
+excess = net inflow).  Runs every variant (data, topo, pp) under the three drivers
+(persistent one-launch solve, stepped per-phase launches, stepped with the BFS
+levels replayed from captured graphs)."""
 import os
 import sys
 

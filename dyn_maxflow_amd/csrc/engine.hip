@@ -187,7 +187,8 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
 
 constexpr int kNoGraph = 1;  // internal: capture refused, use per-level launches
 
-// Captures a segment of 2^idx (level, advance) pairs for BFS mode `mode`.
+// Captures a segment of 2^idx level launches (idx >= 1, so every segment
+// starts and ends on cursor slot 0) for BFS mode `mode`.
 int capture_segment(dmf_engine* E, int mode, int idx) {
   if (E->seg_exec[mode][idx]) return DMF_OK;
   // fixed at capture: at most 8 blocks per CU (a full grid makes every empty
@@ -197,10 +198,8 @@ int capture_segment(dmf_engine* E, int mode, int idx) {
   if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
   bool ok = true;
   for (int i = 0; i < (1 << idx) && ok; ++i) {
-    hipLaunchKernelGGL(k_bfs_level_dev, dim3(grid), dim3(kBlock), 0, st, E->d_dev, mode);
+    hipLaunchKernelGGL(k_bfs_level_dev, dim3(grid), dim3(kBlock), 0, st, E->d_dev, mode, i & 1);
     ok = hipGetLastError() == hipSuccess;
-    hipLaunchKernelGGL(k_bfs_advance, dim3(1), dim3(kWave), 0, st, E->d_dev);
-    ok = ok && hipGetLastError() == hipSuccess;
   }
   hipGraph_t g = nullptr;
   const bool ended = hipStreamEndCapture(st, &g) == hipSuccess;
@@ -237,13 +236,13 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
   int rc;
   if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) return rc;
   if ((rc = zero_dev(E, D.ctrl->wq, 3 * sizeof(unsigned long long)))) return rc;
-  if ((rc = zero_dev(E, &D.ctrl->blo, 2 * sizeof(int32_t)))) return rc;
+  if ((rc = zero_dev(E, D.ctrl->bcur, sizeof(D.ctrl->bcur)))) return rc;
   hipLaunchKernelGGL(k_store_dev, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev);
   ++E->launches;
   HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
   else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode);
-  int idx = 0;
+  int idx = 1;
   while (idx + 1 < dmf_engine::kSegSizes && (1 << idx) < E->seg_levels[mode] + 1) ++idx;
   int lvl = 0;
   for (;;) {
@@ -251,7 +250,7 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
     HIP_OK(hipGraphLaunch(E->seg_exec[mode][idx], E->stream));
     ++E->launches;
     if ((rc = sync_ctrl(E))) return rc;
-    lvl = E->h_ctrl->blvl;
+    lvl = E->h_ctrl->bcur[0][1];
     if (E->h_ctrl->qcnt[lvl % 3] == 0) break;
     if (lvl > E->n + 1) return DMF_EDEVICE;
     if (idx + 1 < dmf_engine::kSegSizes) ++idx;
@@ -260,7 +259,7 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
   E->last.bfs_levels += lvl;
   E->last.global_relabels += 1;
   E->last_levels = lvl;
-  *visited = E->h_ctrl->blo;
+  *visited = E->h_ctrl->bcur[0][0];
   return DMF_OK;
 }
 

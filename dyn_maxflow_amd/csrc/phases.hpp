@@ -73,8 +73,8 @@ struct Ctrl {
   int32_t visited;    // vertices labelled by the last global relabel
   int32_t rounds;     // rounds of the last solve
   int32_t inc;        // updates that raise an arc's capacity (scheduler feature)
-  int32_t blo;        // graph-replayed BFS: first order[] index of the current level
-  int32_t blvl;       // graph-replayed BFS: current level
+  int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
+                      // read from slot r and written to slot r^1 by each level
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
   long long flow;
 };
@@ -1442,20 +1442,33 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
   t.flush(D.ctrs);
 }
 // Graph-replayed BFS levels.  A captured segment is a fixed chain of
-// (k_bfs_level_dev, k_bfs_advance) pairs; the frontier of the current level is
-// found from the device cursor (ctrl->blo, ctrl->blvl) rather than from kernel
-// arguments, and once a level comes out empty every later pair of the segment
-// returns at once.  The engine state is read through a device copy of Dev
-// (k_store_dev) because captured kernel arguments are frozen at capture time.
+// k_bfs_level_dev launches whose cursor slot alternates 0,1,0,1,...; the
+// frontier of the current level is found from the device cursor rather than
+// from kernel arguments, and once a level comes out empty every later launch
+// of the segment returns at once.  The engine state is read through a device
+// copy of Dev (k_store_dev) because captured kernel arguments are frozen at
+// capture time.
 __global__ void k_store_dev(Dev D, Dev* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = D;
 }
-__global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, int mode) {
+// Reads cursor slot r.  Block 0 writes the next level's cursor into slot r^1,
+// which no block of this launch reads, and clears ring slot (lvl+2)%3 (the
+// size of level lvl-1, already folded into the cursor; level lvl+1 appends
+// to it).
+__global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, int mode, int r) {
   const Dev D = *pD;
-  const int lvl = ldr(&D.ctrl->blvl);
+  const int lo = ldr(&D.ctrl->bcur[r][0]);
+  const int lvl = ldr(&D.ctrl->bcur[r][1]);
   const int cnt = ldr(&D.ctrl->qcnt[lvl % 3]);
+  if (is_block0_thread0()) {
+    D.ctrl->bcur[r ^ 1][0] = lo + cnt;
+    D.ctrl->bcur[r ^ 1][1] = cnt ? lvl + 1 : lvl;
+    if (cnt) {
+      D.ctrl->qcnt[(lvl + 2) % 3] = 0;
+      D.ctrl->wq[(lvl + 2) % 3] = 0;
+    }
+  }
   if (cnt == 0) return;  // uniform: the BFS is over
-  const int lo = ldr(&D.ctrl->blo);
   Tally t;
   const unsigned long long wpk = ldru64(&D.ctrl->wq[lvl % 3]);
   if (use_bottom_up(D, mode, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
@@ -1465,20 +1478,6 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
     ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
   }
   t.flush(D.ctrs);
-}
-// One thread: moves the cursor past the level just expanded.  Slot (lvl+2)%3
-// held the frontier size of level lvl-1 (already folded into blo) and is the
-// one level lvl+1 appends to.
-__global__ void k_bfs_advance(const Dev* __restrict__ pD) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Ctrl* c = pD->ctrl;
-  const int lvl = c->blvl;
-  const int cnt = c->qcnt[lvl % 3];
-  if (cnt == 0) return;
-  c->qcnt[(lvl + 2) % 3] = 0;
-  c->wq[(lvl + 2) % 3] = 0;
-  c->blo += cnt;
-  c->blvl = lvl + 1;
 }
 __global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
   Tally t;

@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_ARC, BYTES_VERTEX, BYTES_UPDATE = 12, 16, 24
+# HBM traffic of the dominant kernel measured with rocprofv3 --pmc (FETCH_SIZE,
+# WRITE_SIZE per dispatch, scaled by a plain copy of known size); written from
+# the committed profile summary of this workload
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bfs_level_rmat20_traffic.json")
 
 
 def log(*a):
@@ -94,8 +98,10 @@ class HipBuffers:
         self.ptrs = []
 
 
-def run_gpu(g, s, t, batches, params, steps, warmup, label):
-    """Returns per-batch records for `steps` timed batches after `warmup` untimed."""
+def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=None):
+    """Returns per-batch records for `steps` timed batches after `warmup` untimed.
+    With profile, every BFS level launch is timed by HIP events and the BFS
+    counters of every solve (static ones included) are summed into prof."""
     import dyn_maxflow_amd as dmf
     bufs = HipBuffers()
     dev = [tuple(bufs.upload(a) for a in b) for b in batches]
@@ -104,9 +110,16 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label):
     done = 0
     total = steps + warmup
     with dmf.Engine(g, s, t) as eng:
+        eng.set_profiling(profile)
+
+        def account(c):
+            if prof is not None:
+                for key in ("bfs_level_ms", "bfs_level_launches", "bfs_arc_visits", "bfs_vertex_visits"):
+                    prof[key] = prof.get(key, 0) + c[key]
         while done < total:
             eng.reset()
             eng.static_solve(params)
+            account(eng.counters())
             for bi, (ds, dd, dw) in enumerate(dev):
                 if done >= total:
                     break
@@ -115,6 +128,7 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label):
                 f = eng.apply_batch_device(len(batches[bi][0]), ds, dd, dw, params)
                 wall = (time.perf_counter() - w0) * 1e3
                 c = eng.counters()
+                account(c)
                 if bi in flows and flows[bi] != f:
                     raise RuntimeError(f"{label}: batch {bi} flow {f} differs from earlier repeat {flows[bi]}")
                 flows[bi] = f
@@ -195,6 +209,59 @@ def cpu_baseline(g, s, t, batches, kc, seconds_cap):
     }
 
 
+def roofline(prof):
+    """Dominant kernel (k_bfs_level, the global relabel's level expansion) of
+    the plain path: algorithmic bytes per launch (12 B per arc visit + 16 B per
+    vertex visit, device counters) / mean launch duration (HIP events stamped
+    by each launch on the engine's stream), over every level launch of the
+    profiled pass."""
+    n = prof.get("bfs_level_launches", 0)
+    ms = prof.get("bfs_level_ms", 0.0)
+    if not n or ms <= 0:
+        return None
+    bytes_alg = BYTES_ARC * prof["bfs_arc_visits"] + BYTES_VERTEX * prof["bfs_vertex_visits"]
+    gbs = bytes_alg / (ms / 1e3) / 1e9
+    tr = traffic_from_profile()
+    return {
+        "bound": "hbm",
+        "kernel": "k_bfs_level",
+        "achieved": round(gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(gbs / HBM_PEAK_GBS, 4),
+        "traffic": tr["traffic_bytes_per_launch"] if tr else None,
+        "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None,
+        "bytes_alg_per_launch": round(bytes_alg / n),
+        "avg_launch_us": round(ms * 1e3 / n, 2),
+        "launches": n,
+        "pass": "profiled plain pass: static solves and batches, every level launch",
+    }
+
+
+def copy_probe(mib):
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
+    give the scale of those counters."""
+    bufs = HipBuffers()
+    n = mib << 18  # int32 elements
+    a = bufs.upload(np.arange(n, dtype=np.int32))
+    b = bufs.upload(np.zeros(n, dtype=np.int32))
+    for _ in range(5):
+        if bufs.hip.hipMemcpy(bufs.C.c_void_p(b), bufs.C.c_void_p(a), n * 4, 3) != 0:  # device -> device
+            raise RuntimeError("hipMemcpy failed")
+    bufs.sync()
+    bufs.free()
+    print(json.dumps({"copy_probe_bytes": n * 4, "copies": 5}), flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per k_bfs_level launch from the committed PMC summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,7 +277,13 @@ def main():
                     help="time the learned scheduler beside the plain fixed schedule")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--roofline-steps", type=int, default=30,
+                    help="batches of the profiled plain pass that measures the dominant kernel")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the profiled pass (the command its rocprofv3 summary is taken from)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (PMC scaling reference) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -221,9 +294,19 @@ def main():
     if dmf.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     g, s, t, _, batches = make_workload(args.scale, args.ef, args.seed, 10)
     kc = max(1, g.m // g.n)
     plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
+    prof = {}
+    precs, _ = run_gpu(g, s, t, batches, plain, args.roofline_steps, 2, "profiled plain", profile=True, prof=prof)
+    roof = roofline(prof)
+    if args.roofline_only:
+        print(json.dumps({"roofline": roof, "profiled_p50_ms": nearest_rank([r["device_ms"] for r in precs], 50)}),
+              flush=True)
+        return
     recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain")
     S = summarize(recs)
     L = None
@@ -270,16 +353,10 @@ def main():
                   "roofline_frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4)},
         "learned_speedup_p50": round(S["p50_ms"] / L["p50_ms"], 3) if L is not None else None,
         "flows_verified_vs_dinic": verified,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "k_solve_persistent" if args.mode == 0 else "all phase kernels",
-            "achieved": round(S["achieved_gbs"], 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_alg_per_launch": round(S["bytes_per_batch"]),
-        },
+        "roofline": roof,
+        "all_kernels": {"achieved_gbs": round(S["achieved_gbs"], 2),
+                        "frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                        "bytes_alg_per_batch": round(S["bytes_per_batch"])},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -88,6 +88,10 @@ class Counters(C.Structure):
         ("atomics", C.c_uint64),
         ("launches", C.c_uint64),
         ("device_ms", C.c_double),
+        ("bfs_arc_visits", C.c_uint64),
+        ("bfs_vertex_visits", C.c_uint64),
+        ("bfs_level_launches", C.c_uint64),
+        ("bfs_level_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -213,6 +217,7 @@ def _lib():
         "dmf_batch_apply": (C.c_int, [C.c_void_p, _I32P, _I32P, _I32P, _I32P, C.POINTER(Params), _I64P]),
         "dmf_batch_flows": (C.c_int, [C.c_void_p, _I64P]),
         "dmf_engine_footprint": (C.c_int64, [C.c_int32, C.c_int32]),
+        "dmf_engine_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
         "dmf_engine_reserve_updates": (C.c_int, [C.c_void_p, C.c_int32]),
         "dmf_server_config_default": (None, [C.POINTER(ServerConfig)]),
         "dmf_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
@@ -478,6 +483,10 @@ class Engine:
 
     def set_stream(self, stream_ptr: int) -> None:
         _check(_lib().dmf_engine_set_stream(self._h, C.c_void_p(stream_ptr)), "set_stream")
+
+    def set_profiling(self, on: bool = True) -> None:
+        """Times every BFS level launch with HIP events (counters bfs_level_ms/_launches)."""
+        _check(_lib().dmf_engine_set_profiling(self._h, 1 if on else 0), "set_profiling")
 
     def reserve_updates(self, k: int) -> None:
         _check(_lib().dmf_engine_reserve_updates(self._h, k), "reserve_updates")

@@ -22,6 +22,7 @@
 //                              reference's scan of the tail's adjacency
 //   wq_vert/wq_base            wide-vertex chunk queues; wide_vert/wide_base
 //                              the static list of every vertex of degree > 1024
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -84,6 +85,9 @@ struct dmf_engine {
   bool use_graph = false;
   bool graph_failed = false;
   int seg_levels[3] = {0, 0, 0};
+  // per-launch BFS level timing (dmf_engine_set_profiling)
+  bool profile = false;
+  hipEvent_t pev0 = nullptr, pev1 = nullptr;
 };
 
 namespace {
@@ -166,13 +170,25 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
   int lo = 0, lvl = 0;
   for (;;) {
     if ((rc = sync_ctrl(E))) return rc;
+    if (E->profile && lvl > 0) {  // the previous level's launch has completed
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, E->pev0, E->pev1));
+      E->last.bfs_level_ms += ms;
+      E->last.bfs_level_launches += 1;
+    }
     const int32_t cnt = E->h_ctrl->qcnt[lvl % 3];
     const unsigned long long wpk = E->h_ctrl->wq[lvl % 3];
     if (cnt == 0) break;
-    if (use_bottom_up(D, mode, cnt, chunks_of(wpk))) {
-      LAUNCH(k_bfs_level, blocks_for(E->n), D, mode, lo, cnt, lvl, wpk, 1);
+    const int bu = use_bottom_up(D, mode, cnt, chunks_of(wpk)) ? 1 : 0;
+    const int grid = bu ? blocks_for(E->n) : bfs_blocks(cnt, chunks_of(wpk));
+    if (E->profile) {
+      // events stamped by the launch itself: the span is the kernel's execution
+      hipExtLaunchKernelGGL(k_bfs_level, dim3(grid), dim3(kBlock), 0, E->stream, E->pev0, E->pev1, 0, D, mode, lo,
+                            cnt, lvl, wpk, bu);
+      ++E->launches;
+      HIP_OK(hipGetLastError());
     } else {
-      LAUNCH(k_bfs_level, bfs_blocks(cnt, chunks_of(wpk)), D, mode, lo, cnt, lvl, wpk, 0);
+      LAUNCH(k_bfs_level, grid, D, mode, lo, cnt, lvl, wpk, bu);
     }
     lo += cnt;
     ++lvl;
@@ -247,9 +263,17 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
   int lvl = 0;
   for (;;) {
     if ((rc = capture_segment(E, mode, idx))) return rc;
+    if (E->profile) HIP_OK(hipEventRecord(E->pev0, E->stream));
     HIP_OK(hipGraphLaunch(E->seg_exec[mode][idx], E->stream));
+    if (E->profile) HIP_OK(hipEventRecord(E->pev1, E->stream));
     ++E->launches;
     if ((rc = sync_ctrl(E))) return rc;
+    if (E->profile) {
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, E->pev0, E->pev1));
+      E->last.bfs_level_ms += ms;
+      E->last.bfs_level_launches += static_cast<uint64_t>(1) << idx;
+    }
     lvl = E->h_ctrl->bcur[0][1];
     if (E->h_ctrl->qcnt[lvl % 3] == 0) break;
     if (lvl > E->n + 1) return DMF_EDEVICE;
@@ -555,6 +579,8 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
   if (ctr[C_ROUNDS]) E->last.rounds = ctr[C_ROUNDS];
   E->last.updates = ctr[C_UPD] ? ctr[C_UPD] : E->last.updates;
   E->last.atomics = ctr[C_ATOMIC];
+  E->last.bfs_arc_visits = ctr[C_BFSARC];
+  E->last.bfs_vertex_visits = ctr[C_BFSVERT];
   E->last.launches = E->launches;
   E->last.device_ms = ms;
   if (c.missing) return DMF_ENOARC;
@@ -910,6 +936,8 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(E->d_sched);
   if (E->trace) std::fclose(E->trace);
   if (E->ev0) hipEventDestroy(E->ev0);
+  if (E->pev0) (void)hipEventDestroy(E->pev0);
+  if (E->pev1) (void)hipEventDestroy(E->pev1);
   if (E->ev1) hipEventDestroy(E->ev1);
   if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
   delete E;
@@ -1039,6 +1067,16 @@ int dmf_engine_counters(dmf_engine* E, dmf_counters* c) {
 }
 
 int64_t dmf_engine_device_bytes(const dmf_engine* E) { return E ? E->bytes : 0; }
+
+int dmf_engine_set_profiling(dmf_engine* E, int on) {
+  if (!E) return DMF_EINVAL;
+  if (on && !E->pev0) {
+    HIP_OK(hipEventCreate(&E->pev0));
+    HIP_OK(hipEventCreate(&E->pev1));
+  }
+  E->profile = on != 0;
+  return DMF_OK;
+}
 
 int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   if (n < 0 || m < 0) return 0;

@@ -52,6 +52,8 @@ enum CtrIdx {
   C_ROUNDS,
   C_UPD,
   C_ATOMIC,
+  C_BFSARC,   // arc visits of BFS level launches (also counted in C_ARC)
+  C_BFSVERT,  // vertex visits of BFS level launches (also counted in C_VERT)
   C_N = 16
 };
 
@@ -217,7 +219,9 @@ __device__ __forceinline__ void ctr_add(unsigned long long* ctrs, int idx, unsig
 
 struct Tally {
   unsigned long long arc = 0, vert = 0, push = 0, relabel = 0, repair = 0, atom = 0;
-  __device__ void flush(unsigned long long* ctrs) {
+  // bfs: the block is a BFS level launch; its arc and vertex visits are also
+  // added to the BFS-only counters
+  __device__ void flush(unsigned long long* ctrs, bool bfs = false) {
     __shared__ unsigned long long red[6][kBlock / kWave];
     unsigned long long v[6] = {arc, vert, push, relabel, repair, atom};
     const int wv = threadIdx.x / kWave;
@@ -229,10 +233,11 @@ struct Tally {
       if (lane_id() == 0) red[i][wv] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
-      const int idx[6] = {C_ARC, C_VERT, C_PUSH, C_RELABEL, C_REPAIR, C_ATOMIC};
+    if (threadIdx.x < 8 && (threadIdx.x < 6 || bfs)) {
+      const int idx[8] = {C_ARC, C_VERT, C_PUSH, C_RELABEL, C_REPAIR, C_ATOMIC, C_BFSARC, C_BFSVERT};
+      const int src = threadIdx.x < 6 ? threadIdx.x : threadIdx.x - 6;
       unsigned long long sum = 0;
-      for (int w = 0; w < kBlock / kWave; ++w) sum += red[threadIdx.x][w];
+      for (int w = 0; w < kBlock / kWave; ++w) sum += red[src][w];
       if (sum) ctr_add(ctrs, idx[threadIdx.x], sum);
     }
     __syncthreads();
@@ -1439,7 +1444,7 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
     const WList W = wlist(D, lvl % 3, wpk);
     ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
   }
-  t.flush(D.ctrs);
+  t.flush(D.ctrs, true);
 }
 // Graph-replayed BFS levels.  A captured segment is a fixed chain of
 // k_bfs_level_dev launches whose cursor slot alternates 0,1,0,1,...; the
@@ -1477,7 +1482,7 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
     const WList W = wlist(D, lvl % 3, wpk);
     ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
   }
-  t.flush(D.ctrs);
+  t.flush(D.ctrs, true);
 }
 __global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
   Tally t;

@@ -3,9 +3,10 @@
   python -m dyn_maxflow_amd.sched_train collect --out DIR   (needs the GPU)
   python -m dyn_maxflow_amd.sched_train train --traces DIR  (CPU; writes csrc/sched_weights.inc)
 
-collect: for a set of training workloads (R-MAT scale 15-18 and small grids,
-several batch sizes and increase/decrease mixes, none of them the RMAT-20
-bench graph) every batch is solved under every fixed schedule in
+collect: for a set of training workloads (R-MAT scale 15-20 with seeds 11-16
+and small grids, several batch sizes and increase/decrease mixes; the RMAT-20
+bench graph, seed 2, is not among them) every batch is solved under every
+fixed schedule in
 {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}.  The engine writes one
 trace line per scheduler evaluation (the 8 device-computed features, the round,
 whether that round started with a global relabel) and one summary line per
@@ -39,7 +40,7 @@ INTERVALS = [1, 2, 3, 5, 8]
 def _workloads():
     import dyn_maxflow_amd as dmf
     out = []
-    for scale, seed in ((15, 11), (16, 12), (17, 13), (18, 14)):
+    for scale, seed in ((15, 11), (16, 12), (17, 13), (18, 14), (19, 15), (20, 16)):
         u, v, w = dmf.gen_rmat(scale, 16, seed)
         out.append((f"rmat{scale}s{seed}", u, v, w, 1, 100))
     for side, seed in ((128, 21), (256, 22)):

@@ -165,6 +165,10 @@ typedef struct dmf_counters {
   uint64_t atomics;        /* global atomic RMW operations issued          */
   uint64_t launches;       /* kernel launches                              */
   double device_ms;        /* HIP-event span of the solve on its stream    */
+  uint64_t bfs_arc_visits;     /* arc visits of the BFS level launches       */
+  uint64_t bfs_vertex_visits;  /* vertex visits of the BFS level launches    */
+  uint64_t bfs_level_launches; /* BFS level launches timed (profiling on)    */
+  double bfs_level_ms;         /* their summed HIP-event durations           */
 } dmf_counters;
 
 typedef struct dmf_engine dmf_engine;
@@ -217,6 +221,12 @@ int dmf_engine_read_state(dmf_engine* e, int32_t* res, int32_t* cap, int64_t* ex
                           int32_t* height);
 
 int dmf_engine_counters(dmf_engine* e, dmf_counters* c);
+
+/* Per-launch timing of the BFS level kernel: with on != 0, the stepped
+ * driver brackets every level launch with HIP events on the engine's stream
+ * (bfs_level_ms, bfs_level_launches in dmf_counters; graph-replayed levels
+ * are timed per segment). */
+int dmf_engine_set_profiling(dmf_engine* e, int on);
 
 /* Bytes of device memory held by this engine. */
 int64_t dmf_engine_device_bytes(const dmf_engine* e);

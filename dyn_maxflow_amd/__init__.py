@@ -92,6 +92,7 @@ class Counters(C.Structure):
         ("bfs_vertex_visits", C.c_uint64),
         ("bfs_level_launches", C.c_uint64),
         ("bfs_level_ms", C.c_double),
+        ("variant", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -70,6 +70,11 @@ struct dmf_engine {
   SchedState* h_sched = nullptr;  // pinned mirror
   int last_levels = 0;
   int last_variant = -1;          // variant of the last solve (-1 = none yet)
+  // The push-pull batch continues from the state of a push-pull static solve
+  // (sink saturated, cut in side[]); after a data or topology static solve it
+  // would end short of the maximum (the reference's pp always starts from its
+  // own static solve).  Data and topology batches keep the state usable.
+  bool pp_ready = false;
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
   int32_t ninst = 1;              // independent instances stored as one disjoint union
   std::vector<int64_t> voff;      // vertex offset of each instance (batched engines)
@@ -358,11 +363,13 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       return DMF_OK;
     }
     if (consult) {
-      // batch start: the scheduler picks the variant (push-pull needs the cut
-      // of a previous solve, which every solve leaves in side[])
+      // batch start: the scheduler picks the variant (push-pull only from a
+      // push-pull static state)
       if ((rc = run_sched(E, 0, 0, "batch", 0, C.variant, 0))) return rc;
-      if (learned && E->h_sched->valid && E->last_variant >= 0) C.variant = E->h_sched->variant;
+      if (learned && E->h_sched->valid && (E->h_sched->variant != DMF_VARIANT_PP || E->pp_ready))
+        C.variant = E->h_sched->variant;
     }
+    if (C.variant == DMF_VARIANT_PP && !E->pp_ready) C.variant = DMF_VARIANT_DATA;
   }
   const bool pp = C.variant == DMF_VARIANT_PP;
   const bool topo = C.variant == DMF_VARIANT_TOPO;
@@ -426,8 +433,9 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
       }
     }
-    // every solve leaves a cut in side[] so that a push-pull batch may follow
-    if (conv) LAUNCH(k_find_cut, blocks_for(n), D);
+    // the push-pull static solve leaves its cut in side[] (find_cut,
+    // code/dyn_pp_bench.cu:231); other variants leave side[] alone
+    if (conv && pp) LAUNCH(k_find_cut, blocks_for(n), D);
   } else {
     // push-pull batch; stale two-sided labels stay valid between relabels and
     // convergence is only accepted on fresh ones (part_final reads them)
@@ -583,6 +591,7 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
   E->last.bfs_vertex_visits = ctr[C_BFSVERT];
   E->last.launches = E->launches;
   E->last.device_ms = ms;
+  E->last.variant = static_cast<uint64_t>(E->last_variant);
   if (c.missing) return DMF_ENOARC;
   if (c.invalid) return DMF_EINVAL;
   if (rc_driver) return rc_driver;
@@ -592,8 +601,9 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
 }
 
 int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
-  const Cfg C = make_cfg(E, p, entry);
+  Cfg C = make_cfg(E, p, entry);
   if (C.variant < 0 || C.variant > 2 || C.scheduler < 0 || C.scheduler > 1) return DMF_EINVAL;
+  if (entry == 1 && C.variant == DMF_VARIANT_PP && !E->pp_ready) C.variant = DMF_VARIANT_DATA;
   int rc = begin_solve(E);
   if (rc) return rc;
   HIP_OK(hipEventRecord(E->ev0, E->stream));
@@ -608,6 +618,7 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   HIP_OK(hipEventRecord(E->ev1, E->stream));
   if (entry == 1) E->last.updates = static_cast<uint64_t>(E->D.k);
   rc = finish_solve(E, rc_driver, flow);
+  if (entry == 0) E->pp_ready = rc == DMF_OK && E->last_variant == DMF_VARIANT_PP;
   if (E->trace) {
     std::fprintf(E->trace,
                  "{\"solve\":%llu,\"what\":\"summary\",\"entry\":%d,\"variant\":%d,\"scheduler\":%d,"
@@ -967,6 +978,7 @@ int dmf_engine_reset(dmf_engine* E) {
   HIP_OK(hipMemsetAsync(D.excess, 0, sizeof(long long) * E->n, E->stream));
   HIP_OK(hipMemsetAsync(D.side, 0, E->n, E->stream));
   HIP_OK(hipStreamSynchronize(E->stream));
+  E->pp_ready = false;
   return DMF_OK;
 }
 

@@ -501,22 +501,6 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
   }
 }
 
-// Whether BFS level expansion may label w through arc e of frontier vertex v.
-__device__ __forceinline__ bool bfs_arc_ok(const Dev& D, int mode, bool vB, int e, int w) {
-  if (mode == BFS_SINK) return D.rres[e] > 0;
-  if (mode == BFS_PP) {
-    if (is_term(D, w)) return false;
-    // both residuals are read and the value selected: selecting between the
-    // two array fields per lane would force the by-value kernel argument into
-    // per-lane scratch memory
-    const int32_t back = D.rres[e], fwd = D.res[e];
-    return (vB ? back : fwd) > 0;
-  }
-  // BFS_FINAL: non-B neighbours of the stranded part only
-  if (D.side[w] || !D.pfinal[w]) return false;
-  return D.rres[e] > 0;
-}
-
 // A newly labelled vertex: wide ones also enter the next level's wide queue.
 __device__ __forceinline__ bool is_wide(const Dev& D, int v) {
   return (D.widebits[static_cast<unsigned>(v) >> 5] >> (v & 31)) & 1u;
@@ -534,6 +518,34 @@ __device__ __forceinline__ bool bfs_claim(const Dev& D, int w, int nb) {
   return atomicCAS(&D.height[w], D.n, nb) == D.n;
 }
 
+// Whether a BFS level may label head w through arc e of frontier vertex v is
+// split in two, so that the residual can be loaded a tile ahead of its use:
+// bfs_resid reads the residual that decides the arc (BFS_SINK and BFS_FINAL:
+// the reverse residual; BFS_PP: reverse on the B side, forward on the A side),
+// bfs_head_ok checks the head (BFS_PP: never a terminal; BFS_FINAL: a non-B
+// vertex of the stranded part).
+__device__ __forceinline__ int bfs_resid(const Dev& D, int mode, bool vB, int e) {
+  if (mode == BFS_PP) {
+    // both residuals are read and the value selected: selecting between the
+    // two array fields per lane would force the by-value kernel argument into
+    // per-lane scratch memory
+    const int32_t back = D.rres[e], fwd = D.res[e];
+    return vB ? back : fwd;
+  }
+  return D.rres[e];
+}
+__device__ __forceinline__ bool bfs_head_ok(const Dev& D, int mode, int w) {
+  if (mode == BFS_SINK) return true;
+  if (mode == BFS_PP) return !is_term(D, w);
+  return !D.side[w] && D.pfinal[w];
+}
+
+// One pair of 64-arc tiles with their heads and residuals loaded.
+struct BfsTiles {
+  int wA, wB, rA, rB;
+  bool inA, inB;
+};
+
 // Top-down BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
 // code/dyn_pp.h:99-120; final_bfs, code/dyn_pp.h:123-140).  Frontier is
 // order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
@@ -542,35 +554,50 @@ __device__ __forceinline__ bool bfs_claim(const Dev& D, int w, int nb) {
 // at a time are expanded as one flattened arc range, so consecutive lanes read
 // consecutive arcs (coalesced col/residual loads) and each lane finds its
 // arc's tail by a 4-step binary search over the group's degree prefix sums.
-// Both loops keep two 64-arc tiles in flight per wave.
+// Both loops work on pairs of 64-arc tiles and load the next pair's heads and
+// residuals before testing and claiming the current pair's, so the streaming
+// loads overlap the height gathers and the claims.
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
                              int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
   Stage st{sbuf[threadIdx.x / kWave], 0};
   const int nb = lvl + 1;
   int32_t* out = D.order + lo + cnt;
+  auto expand = [&](const BfsTiles& T) {
+    bool cA = T.inA && T.rA > 0 && bfs_head_ok(D, mode, T.wA);
+    bool cB = T.inB && T.rB > 0 && bfs_head_ok(D, mode, T.wB);
+    cA = cA && bfs_unlabelled(D, T.wA);
+    cB = cB && bfs_unlabelled(D, T.wB);
+    t.arc += (T.inA ? 1 : 0) + (T.inB ? 1 : 0);
+    t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
+    const bool hitA = cA && bfs_claim(D, T.wA, nb);
+    const bool hitB = cB && bfs_claim(D, T.wB, nb);
+    bfs_note_wide(D, hitA, T.wA, qnext);
+    bfs_note_wide(D, hitB, T.wB, qnext);
+    st.push(hitA, T.wA, next_cnt, out);
+    st.push(hitB, T.wB, next_cnt, out);
+  };
   // wide frontier vertices: chunks spread over every wave of the grid
   for (int c = r.wid; c < W.chunks; c += r.nw) {
     int a, b;
     const int v = chunk_range(D, W, c, &a, &b);
     const bool vB = mode == BFS_PP && D.side[v] != 0;
-    for (int j0 = a; j0 < b; j0 += 2 * kWave) {
+    auto fetch = [&](int j0) {
+      BfsTiles T;
       const int eA = j0 + lane_id(), eB = eA + kWave;
-      const bool inA = eA < b, inB = eB < b;
-      const int wA = inA ? D.col[eA] : 0;
-      const int wB = inB ? D.col[eB] : 0;
-      bool cA = inA && bfs_arc_ok(D, mode, vB, eA, wA);
-      bool cB = inB && bfs_arc_ok(D, mode, vB, eB, wB);
-      cA = cA && bfs_unlabelled(D, wA);
-      cB = cB && bfs_unlabelled(D, wB);
-      t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
-      t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
-      const bool hitA = cA && bfs_claim(D, wA, nb);
-      const bool hitB = cB && bfs_claim(D, wB, nb);
-      bfs_note_wide(D, hitA, wA, qnext);
-      bfs_note_wide(D, hitB, wB, qnext);
-      st.push(hitA, wA, next_cnt, out);
-      st.push(hitB, wB, next_cnt, out);
+      T.inA = eA < b;
+      T.inB = eB < b;
+      T.wA = T.inA ? D.col[eA] : 0;
+      T.wB = T.inB ? D.col[eB] : 0;
+      T.rA = T.inA ? bfs_resid(D, mode, vB, eA) : 0;
+      T.rB = T.inB ? bfs_resid(D, mode, vB, eB) : 0;
+      return T;
+    };
+    BfsTiles cur = fetch(a);
+    for (int j0 = a; j0 < b; j0 += 2 * kWave) {
+      const BfsTiles now = cur;
+      if (j0 + 2 * kWave < b) cur = fetch(j0 + 2 * kWave);
+      expand(now);
     }
   }
   for (int base = r.wid * kGroup; base < cnt; base += r.nw * kGroup) {
@@ -588,7 +615,8 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     const int incl = wave_incl_scan_i32(deg);
     const int total = __shfl(incl, kWave - 1);
     const int excl = incl - deg;
-    for (int k0 = 0; k0 < total; k0 += 2 * kWave) {
+    // called by the whole wave (shuffles) with a wave-uniform k0
+    auto fetch = [&](int k0) {
       const int pA = k0 + lane_id(), pB = pA + kWave;
       int posA = 0, posB = 0;  // owner lanes: number of group lanes whose prefix end <= p
 #pragma unroll
@@ -601,21 +629,21 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
       const int eA = __shfl(b, posA) + (pA - __shfl(excl, posA));
       const int eB = __shfl(b, posB) + (pB - __shfl(excl, posB));
       const bool okA = __shfl(vB, posA) != 0, okB = __shfl(vB, posB) != 0;
-      const bool inA = pA < total, inB = pB < total;
-      const int wA = inA ? D.col[eA] : 0;
-      const int wB = inB ? D.col[eB] : 0;
-      bool cA = inA && bfs_arc_ok(D, mode, okA, eA, wA);
-      bool cB = inB && bfs_arc_ok(D, mode, okB, eB, wB);
-      cA = cA && bfs_unlabelled(D, wA);
-      cB = cB && bfs_unlabelled(D, wB);
-      t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
-      t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
-      const bool hitA = cA && bfs_claim(D, wA, nb);
-      const bool hitB = cB && bfs_claim(D, wB, nb);
-      bfs_note_wide(D, hitA, wA, qnext);
-      bfs_note_wide(D, hitB, wB, qnext);
-      st.push(hitA, wA, next_cnt, out);
-      st.push(hitB, wB, next_cnt, out);
+      BfsTiles T;
+      T.inA = pA < total;
+      T.inB = pB < total;
+      T.wA = T.inA ? D.col[eA] : 0;
+      T.wB = T.inB ? D.col[eB] : 0;
+      T.rA = T.inA ? bfs_resid(D, mode, okA, eA) : 0;
+      T.rB = T.inB ? bfs_resid(D, mode, okB, eB) : 0;
+      return T;
+    };
+    if (total == 0) continue;
+    BfsTiles cur = fetch(0);
+    for (int k0 = 0; k0 < total; k0 += 2 * kWave) {
+      const BfsTiles now = cur;
+      if (k0 + 2 * kWave < total) cur = fetch(k0 + 2 * kWave);
+      expand(now);
     }
   }
   st.flush(next_cnt, out);

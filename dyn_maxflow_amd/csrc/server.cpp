@@ -300,6 +300,7 @@ int dmf_server_open(dmf_server* S, const dmf_csr* g, int32_t s, int32_t t, int32
   if (!rc) {
     dmf_params p = ss->params;
     p.scheduler = DMF_SCHED_FIXED;  // the static solve has no batch to schedule
+    p.variant = DMF_VARIANT_PP;     // a push-pull static state lets every variant take the batches
     rc = dmf_engine_static_solve(ss->eng, &p, &f);
   }
   std::lock_guard<std::mutex> lk(S->mu);

@@ -71,7 +71,8 @@ def collect(out_dir: str, quick: bool = False) -> None:
         os.environ["DMF_TRACE"] = trace
         with dmf.Engine(g, s, t) as eng:
             os.environ.pop("DMF_TRACE", None)
-            base = dmf.default_params(g)
+            # push-pull static state: every variant may take the batches
+            base = dmf.default_params(g, variant="pp")
             for variant in VARIANTS:
                 for interval in ([1, 3] if quick else INTERVALS):
                     eng.reset()

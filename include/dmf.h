@@ -132,6 +132,11 @@ int dmf_gen_updates(int64_t k, const int32_t* u, const int32_t* v, int32_t* cur_
 #define DMF_VARIANT_DATA 0  /* worklist-driven       (code/dyn_data.h)  */
 #define DMF_VARIANT_TOPO 1  /* all-vertex rounds     (code/dyn_topo.h)  */
 #define DMF_VARIANT_PP 2    /* push-pull over a cut  (code/dyn_pp.h)    */
+/* A push-pull batch continues from the state of a push-pull static solve
+ * (data and topology batches in between are fine).  After a static solve of
+ * another variant (or a reset) a batch that asks for push-pull runs
+ * data-driven instead; dmf_counters.variant reports the variant that ran.
+ * The learned scheduler picks push-pull only from such a state. */
 
 #define DMF_SCHED_FIXED 0   /* global relabel every `relabel_interval` rounds */
 #define DMF_SCHED_LEARNED 1 /* on-device MLP decides relabel + next variant   */
@@ -169,6 +174,7 @@ typedef struct dmf_counters {
   uint64_t bfs_vertex_visits;  /* vertex visits of the BFS level launches    */
   uint64_t bfs_level_launches; /* BFS level launches timed (profiling on)    */
   double bfs_level_ms;         /* their summed HIP-event durations           */
+  uint64_t variant;            /* variant the solve ran (DMF_VARIANT_*)      */
 } dmf_counters;
 
 typedef struct dmf_engine dmf_engine;

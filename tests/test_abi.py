@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import dyn_maxflow_amd as dmf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -60,3 +62,19 @@ def test_default_params_follow_reference_schedule():
     assert p.kc == g.m // g.n
     assert p.relabel_interval == 1
     assert p.variant == dmf.VARIANT_DATA
+
+
+def test_engine_footprint_bounds_the_arrays():
+    for n, m in ((1000, 16000), (1 << 20, 31401920), (8387651, 258678280)):
+        b = dmf.engine_footprint(n, m)
+        assert b >= 28 * m + 31 * n  # the arc and vertex arrays alone
+        assert b <= 28 * m + 40 * n + 12 * 4 * (2 * m + 16) + 48 * (m // 64 + 1) + (1 << 21)
+    assert dmf.engine_footprint(2000, 32000) > dmf.engine_footprint(1000, 16000)
+
+
+def test_server_refuses_without_device():
+    if dmf.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(dmf.DmfError) as ex:
+        dmf.Server(dmf.server_config())
+    assert ex.value.status == dmf.EDEVICE

@@ -292,3 +292,35 @@ def test_autotune_keeps_results(gpu):
         src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 25, 77)
         f = eng.apply_batch(src, dst, nw, p)
         check_engine(eng, g, caps_after(g, [(src, dst, nw)]), s, t, f, "autotuned batch")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_variant_switching_between_batches(gpu, mode):
+    """Push-pull batches continue from a push-pull static state; from any other
+    static state a push-pull request runs data-driven.  Every flow and S_min
+    equals Dinic's through mixed sequences of variants."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(10, 16, 1)
+    cur = w.copy()
+    bl = [dmf.gen_updates(u, v, cur, len(u) * p // 100, 50 + i) for i, p in enumerate((2, 5, 3, 4, 2))]
+    for static_variant, seq in (("data", ["data", "pp", "topo", "pp", "data"]),
+                                ("topo", ["pp", "pp", "data", "topo", "pp"]),
+                                ("pp", ["data", "pp", "topo", "pp", "data"]),
+                                ("pp", ["pp", "topo", "topo", "pp", "pp"])):
+        say(f"[switch] mode={mode} static={static_variant} seq={seq}")
+        with dmf.Engine(g, s, t) as eng:
+            eng.static_solve(dmf.default_params(g, variant=static_variant, mode=mode))
+            for i, var in enumerate(seq):
+                f = eng.apply_batch(*bl[i], dmf.default_params(g, variant=var, mode=mode))
+                ran = eng.counters()["variant"]
+                want = dmf.VARIANTS[var]
+                if var == "pp" and static_variant != "pp":
+                    want = dmf.VARIANT_DATA
+                assert ran == want, (static_variant, seq, i, ran)
+                check_engine(eng, g, caps_after(g, bl[:i + 1]), s, t, f, f"{static_variant} {seq[:i + 1]}")
+        # the learned scheduler may pick any variant from a push-pull state
+        with dmf.Engine(g, s, t) as eng:
+            eng.static_solve(dmf.default_params(g, variant="pp", mode=mode))
+            for i in range(len(bl)):
+                f = eng.apply_batch(*bl[i], dmf.default_params(g, mode=1, scheduler=dmf.SCHED_LEARNED))
+                check_engine(eng, g, caps_after(g, bl[:i + 1]), s, t, f, f"learned batch {i}")

@@ -1,6 +1,7 @@
 """Input tool (dmf_input) and bench-driver host logic that needs no GPU."""
 import os
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -112,3 +113,21 @@ def test_rmat_n_matches_power_of_two_and_stays_in_range():
         assert w.min() >= 1 and w.max() <= 100
     with pytest.raises(dmf.DmfError):
         dmf.gen_rmat_n(1, 10, 1)
+
+
+@pytest.mark.parametrize("name,bad", [("dmf_multi_bench", "--instances=0"), ("dmf_serve_bench", "--kmax=1"),
+                                      ("dmf_multi_bench", "--bogus"), ("dmf_serve_bench", "--bogus")])
+def test_workload_driver_argument_errors(name, bad):
+    r = subprocess.run([os.path.join(BIN, name), bad], capture_output=True, text=True)
+    assert r.returncode == 2 and ("invalid" in r.stderr or "unknown argument" in r.stderr)
+
+
+def test_workload_drivers_refuse_without_device():
+    if dmf.device_count() > 0:
+        pytest.skip("a device is visible")
+    for name in ("dmf_multi_bench", "dmf_serve_bench"):
+        r = subprocess.run([os.path.join(BIN, name)], capture_output=True, text=True)
+        assert r.returncode == 2 and "no GPU" in r.stderr
+    r = subprocess.run([sys.executable, "-m", "dyn_maxflow_amd.grid_bench", "--side", "8"], capture_output=True,
+                       text=True, cwd=os.path.dirname(os.path.dirname(BIN)))
+    assert r.returncode != 0 and "no HIP device" in r.stderr

@@ -3,9 +3,10 @@
 Workload (one GPU): directed R-MAT scale 20, edge factor 16, a/b/c/d =
 0.57/0.19/0.19/0.05, self-loops and duplicates dropped, seed 2, capacities
 uniform [1,100]; s = max out-degree vertex, t = max in-degree vertex != s.
-One repeat = reset to the original graph, static solve (untimed), then 10
-successive update batches of 1%, 2%, ..., 10% of the input edges (50/50
-increase/decrease).  A step is one batch: apply + repair + re-converge, timed
+One repeat = reset to the original graph, static solve (untimed; push-pull by
+default, for the plain and the learned runs alike, so that the scheduler may
+send a batch to any variant), then 10 successive update batches of 1%, 2%,
+..., 10% of the input edges (50/50 increase/decrease).  A step is one batch: apply + repair + re-converge, timed
 with HIP events on the engine's stream, inputs already in HBM.
 
 Prints one JSON line (see the keys at the bottom).  --gpus > 1 is not
@@ -98,7 +99,7 @@ class HipBuffers:
         self.ptrs = []
 
 
-def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=None):
+def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=None, static_params=None):
     """Returns per-batch records for `steps` timed batches after `warmup` untimed.
     With profile, every BFS level launch is timed by HIP events and the BFS
     counters of every solve (static ones included) are summed into prof."""
@@ -118,7 +119,7 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=
                     prof[key] = prof.get(key, 0) + c[key]
         while done < total:
             eng.reset()
-            eng.static_solve(params)
+            eng.static_solve(static_params if static_params is not None else params)
             account(eng.counters())
             for bi, (ds, dd, dw) in enumerate(dev):
                 if done >= total:
@@ -275,6 +276,9 @@ def main():
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")
+    ap.add_argument("--static-variant", default="pp",
+                    help="variant of the untimed static solve that every repeat starts from (the same for "
+                         "the plain and the learned runs; push-pull lets every variant take the batches)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -300,19 +304,22 @@ def main():
     g, s, t, _, batches = make_workload(args.scale, args.ef, args.seed, 10)
     kc = max(1, g.m // g.n)
     plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
+    static = dmf.default_params(g, variant=args.static_variant, mode=args.mode)
     prof = {}
-    precs, _ = run_gpu(g, s, t, batches, plain, args.roofline_steps, 2, "profiled plain", profile=True, prof=prof)
+    precs, _ = run_gpu(g, s, t, batches, plain, args.roofline_steps, 2, "profiled plain", profile=True, prof=prof,
+                       static_params=static)
     roof = roofline(prof)
     if args.roofline_only:
         print(json.dumps({"roofline": roof, "profiled_p50_ms": nearest_rank([r["device_ms"] for r in precs], 50)}),
               flush=True)
         return
-    recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain")
+    recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain", static_params=static)
     S = summarize(recs)
     L = None
     if args.sched != "plain":
         learned = dmf.default_params(g, variant=args.variant, mode=1, scheduler=dmf.SCHED_LEARNED)
-        lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned")
+        lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned",
+                                static_params=static)
         L = summarize(lrecs)
         if lflows != flows:
             raise RuntimeError("learned scheduler produced different flows from the plain path")
@@ -340,6 +347,7 @@ def main():
                         f"of {g.m // 2 if False else 'input'} edges, 50/50 inc/dec, caps U[1,100]",
             "n": g.n, "m": g.m, "s": s, "t": t,
             "variant": args.variant, "kc": kc, "relabel_interval": args.interval,
+            "static_solve_variant": args.static_variant,
             "driver": "persistent" if args.mode == 0 else "stepped",
         },
         "scheduler": "learned" if L is not None else "fixed",

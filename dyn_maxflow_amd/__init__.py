@@ -62,7 +62,7 @@ class _Csr(C.Structure):
 
 
 class Params(C.Structure):
-    """dmf_params: variant, kc, relabel_interval, scheduler, max_rounds, mode."""
+    """dmf_params: variant, kc, relabel_interval, scheduler, max_rounds, mode, lazy_start."""
 
     _fields_ = [
         ("variant", C.c_int32),
@@ -71,6 +71,7 @@ class Params(C.Structure):
         ("scheduler", C.c_int32),
         ("max_rounds", C.c_int32),
         ("mode", C.c_int32),
+        ("lazy_start", C.c_int32),
     ]
 
 
@@ -386,7 +387,7 @@ def device_count() -> int:
 
 
 def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=0,
-                   max_rounds=0, scheduler=SCHED_FIXED) -> Params:
+                   max_rounds=0, scheduler=SCHED_FIXED, lazy_start=0) -> Params:
     p = Params()
     c = g._csr()
     _lib().dmf_params_default(C.byref(c), C.byref(p))
@@ -397,6 +398,7 @@ def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=0
     p.mode = mode
     p.max_rounds = max_rounds
     p.scheduler = scheduler
+    p.lazy_start = lazy_start
     return p
 
 

@@ -75,6 +75,12 @@ struct dmf_engine {
   // would end short of the maximum (the reference's pp always starts from its
   // own static solve).  Data and topology batches keep the state usable.
   bool pp_ready = false;
+  // Labels the last data/topology solve converged on (a fresh relabel from
+  // the sinks; order[0, visited_last) its labelled vertices).  A batch may
+  // start on them instead of relabelling first: its updates can only make
+  // them stale, and convergence is accepted on fresh labels only.
+  bool labels_fresh = false;
+  int visited_last = 0;
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
   int32_t ninst = 1;              // independent instances stored as one disjoint union
   std::vector<int64_t> voff;      // vertex offset of each instance (batched engines)
@@ -346,6 +352,8 @@ int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int 
 int solve_stepped(dmf_engine* E, const Cfg& C0) {
   Dev& D = E->D;
   int rc;
+  const bool labels_in = E->labels_fresh;
+  E->labels_fresh = false;  // set again only by a converged data-driven loop
   Cfg C = C0;
   const bool learned = C.scheduler == DMF_SCHED_LEARNED;
   const bool consult = learned || E->trace != nullptr;
@@ -389,6 +397,16 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   if (!pp || C.entry == 0) {
     int since = C.interval, visited = 0;
     bool want_relabel = true;  // scheduler decision for the coming round
+    if (C.entry == 1 && labels_in && !topo) {
+      // lazy start (data-driven only: its loop accepts convergence on fresh
+      // labels alone): keep the previous solve's labels for the first round
+      const bool skip = learned ? (E->h_sched->valid && E->h_sched->relabel == 0) : C.lazy != 0;
+      if (skip) {
+        since = 0;
+        visited = E->visited_last;
+        want_relabel = false;
+      }
+    }
     for (;;) {
       bool fresh = false;
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
@@ -436,6 +454,12 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     // the push-pull static solve leaves its cut in side[] (find_cut,
     // code/dyn_pp_bench.cu:231); other variants leave side[] alone
     if (conv && pp) LAUNCH(k_find_cut, blocks_for(n), D);
+    // the data-driven loop ends right after a fresh relabel: its labels and
+    // labelled set order[0, visited) are what a lazy-start batch begins from
+    if (conv && !pp && !topo) {
+      E->labels_fresh = true;
+      E->visited_last = visited;
+    }
   } else {
     // push-pull batch; stale two-sided labels stay valid between relabels and
     // convergence is only accepted on fresh ones (part_final reads them)
@@ -540,6 +564,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
     d.scheduler = DMF_SCHED_FIXED;
     d.max_rounds = 0;
     d.mode = 0;
+    d.lazy_start = 0;
     p = &d;
   }
   c.variant = p->variant;
@@ -548,6 +573,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
   c.max_rounds = p->max_rounds > 0 ? p->max_rounds : 4 * E->n + 1000;
   c.entry = entry;
   c.scheduler = p->scheduler;
+  c.lazy = p->lazy_start;
   return c;
 }
 
@@ -613,6 +639,7 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   const bool stepped = (p && (p->mode == 1 || p->mode == 2)) || E->coop_blocks <= 0 ||
                        C.scheduler == DMF_SCHED_LEARNED || E->trace != nullptr;
   E->use_graph = p && p->mode == 2;
+  if (!stepped) E->labels_fresh = false;
   int rc_driver = stepped ? solve_stepped(E, C) : solve_persistent(E, C);
   if (rc_driver == DMF_EDEVICE) return rc_driver;
   HIP_OK(hipEventRecord(E->ev1, E->stream));
@@ -622,9 +649,9 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   if (E->trace) {
     std::fprintf(E->trace,
                  "{\"solve\":%llu,\"what\":\"summary\",\"entry\":%d,\"variant\":%d,\"scheduler\":%d,"
-                 "\"interval\":%d,\"k\":%d,\"device_ms\":%.4f,\"wall_ms\":%.4f,\"rounds\":%llu,"
+                 "\"interval\":%d,\"lazy\":%d,\"k\":%d,\"device_ms\":%.4f,\"wall_ms\":%.4f,\"rounds\":%llu,"
                  "\"relabels\":%llu,\"rc\":%d}\n",
-                 (unsigned long long)E->solve_id, entry, E->last_variant, C.scheduler, C.interval, E->D.k,
+                 (unsigned long long)E->solve_id, entry, E->last_variant, C.scheduler, C.interval, C.lazy, E->D.k,
                  E->last.device_ms, since_solve_ms(E), (unsigned long long)E->last.rounds,
                  (unsigned long long)E->last.global_relabels, rc);
     std::fflush(E->trace);
@@ -852,7 +879,8 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
   p->relabel_interval = 1;
   p->scheduler = DMF_SCHED_FIXED;
   p->max_rounds = 0;
-  p->mode = 1;  // stepped: fastest single-graph driver on MI355X
+  p->mode = 1;  // stepped: the fastest driver on R-MAT graphs
+  p->lazy_start = 0;
 }
 
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
@@ -979,6 +1007,7 @@ int dmf_engine_reset(dmf_engine* E) {
   HIP_OK(hipMemsetAsync(D.side, 0, E->n, E->stream));
   HIP_OK(hipStreamSynchronize(E->stream));
   E->pp_ready = false;
+  E->labels_fresh = false;
   return DMF_OK;
 }
 

@@ -140,6 +140,7 @@ struct Cfg {
   int32_t max_rounds;
   int32_t entry;  // 0 = static (init + saturate), 1 = batch (apply + repair)
   int32_t scheduler;  // DMF_SCHED_*
+  int32_t lazy;       // batch may start on the labels the previous solve ended with
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;

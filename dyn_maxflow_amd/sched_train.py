@@ -7,15 +7,19 @@ collect: for a set of training workloads (R-MAT scale 15-20 with seeds 11-16
 and small grids, several batch sizes and increase/decrease mixes; the RMAT-20
 bench graph, seed 2, is not among them) every batch is solved under every
 fixed schedule in
-{data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}.  The engine writes one
+{data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}, data-driven ones
+also with the lazy start (no relabel before the first round), each from a
+push-pull static state so that every variant may take the batches.  The engine writes one
 trace line per scheduler evaluation (the 8 device-computed features, the round,
 whether that round started with a global relabel) and one summary line per
 solve (device ms, variant, interval).
 
 train: for every (workload, batch) the fastest schedule is the teacher.  The
 variant head learns that schedule's variant from the batch-start features; the
-relabel head learns, from the features at the end of each round of the
-teacher's run, whether the teacher relabelled before the next round.  A
+relabel head learns whether the teacher relabelled before its first round
+(from the batch-start features; data-driven schedules may start on the labels
+of the previous solve) and, from the features at the end of each round of the
+teacher's run, whether it relabelled before the next round.  A
 numpy MLP 8->32->32->4 (ReLU) is fitted with Adam; the weights are written in
 the row-major SchedWeights order of csrc/sched.hpp.
 """
@@ -75,12 +79,13 @@ def collect(out_dir: str, quick: bool = False) -> None:
             base = dmf.default_params(g, variant="pp")
             for variant in VARIANTS:
                 for interval in ([1, 3] if quick else INTERVALS):
-                    eng.reset()
-                    eng.static_solve(base)
-                    p = dmf.default_params(g, variant=variant, relabel_interval=interval)
-                    for src, dst, nw in batches:
-                        eng.apply_batch(src, dst, nw, p)
-                    print(f"[collect] {name} {variant} interval={interval}", flush=True)
+                    for lazy in ((0, 1) if variant == "data" else (0,)):
+                        eng.reset()
+                        eng.static_solve(base)
+                        p = dmf.default_params(g, variant=variant, relabel_interval=interval, lazy_start=lazy)
+                        for src, dst, nw in batches:
+                            eng.apply_batch(src, dst, nw, p)
+                        print(f"[collect] {name} {variant} interval={interval} lazy={lazy}", flush=True)
 
 
 def _load(trace_dir: str):
@@ -110,7 +115,7 @@ def _examples(data):
             sm = sv.get("summary")
             if not sm or sm["entry"] != 1 or sm["scheduler"] != 0:
                 continue
-            key = (sm["variant"], sm["interval"])
+            key = (sm["variant"], sm["interval"], sm.get("lazy", 0))
             runs.setdefault(key, []).append(sv)
         if not runs:
             continue
@@ -119,10 +124,13 @@ def _examples(data):
             best = min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
             teacher = runs[best][b]
             batch_rows = [r for r in teacher["rows"] if r["what"] == "batch"]
+            rows = [r for r in teacher["rows"] if r["what"] == "round"]
             if batch_rows:
                 Xv.append(batch_rows[0]["feat"])
                 yv.append(best[0])
-            rows = [r for r in teacher["rows"] if r["what"] == "round"]
+                # batch start: did the teacher relabel before its first round?
+                Xr.append(batch_rows[0]["feat"])
+                yr.append(rows[0]["relabelled"] if rows else 1)
             for i, r in enumerate(rows):
                 nxt = rows[i + 1]["relabelled"] if i + 1 < len(rows) else 1
                 Xr.append(r["feat"])

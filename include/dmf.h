@@ -150,6 +150,10 @@ typedef struct dmf_params {
   int32_t mode;             /* 0 = persistent one-launch solve, 1 = stepped,
                                2 = stepped with the BFS levels replayed from
                                captured hipGraph segments                     */
+  int32_t lazy_start;       /* data/topo batches (stepped drivers): skip the
+                               global relabel before the first round when the
+                               previous solve ended on fresh labels.  The
+                               learned scheduler makes this choice itself.    */
 } dmf_params;
 
 /* Fills the reference's fixed schedule: data variant, kc = max(1, m/n),

@@ -324,3 +324,23 @@ def test_variant_switching_between_batches(gpu, mode):
             for i in range(len(bl)):
                 f = eng.apply_batch(*bl[i], dmf.default_params(g, mode=1, scheduler=dmf.SCHED_LEARNED))
                 check_engine(eng, g, caps_after(g, bl[:i + 1]), s, t, f, f"learned batch {i}")
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("interval", [1, 3])
+def test_lazy_start_keeps_results(gpu, mode, interval):
+    """Data-driven batches that start on the previous solve's labels (no
+    relabel before the first round) still end bit-exact."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 9)
+    p = dmf.default_params(g, relabel_interval=interval, mode=mode, lazy_start=1)
+    say(f"[lazy] mode={mode} interval={interval}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(dmf.default_params(g, mode=mode))
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b, pct in enumerate((1, 4, 2, 6)):
+            applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 800 + b))
+            f = eng.apply_batch(*applied[-1], p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"lazy batch {b}")

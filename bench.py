@@ -235,7 +235,7 @@ def roofline(prof):
         "bytes_alg_per_launch": round(bytes_alg / n),
         "avg_launch_us": round(ms * 1e3 / n, 2),
         "launches": n,
-        "pass": "profiled plain pass: static solves and batches, every level launch",
+        "pass": "profiled plain pass on the stepped driver: static solves and batches, every level launch",
     }
 
 
@@ -272,7 +272,8 @@ def main():
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--variant", default="data")
-    ap.add_argument("--mode", type=int, default=1, help="0 persistent one-launch solve, 1 stepped")
+    ap.add_argument("--mode", type=int, default=2,
+                    help="driver: 0 persistent one-launch solve, 1 stepped, 2 stepped with graph-replayed BFS levels")
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")
@@ -305,9 +306,13 @@ def main():
     kc = max(1, g.m // g.n)
     plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
     static = dmf.default_params(g, variant=args.static_variant, mode=args.mode)
+    # The roofline pass runs the plain schedule on the stepped driver: its BFS
+    # level launches (k_bfs_level, the same level expansion that the graph
+    # driver replays) can be timed one by one with launch-stamped events.
     prof = {}
-    precs, _ = run_gpu(g, s, t, batches, plain, args.roofline_steps, 2, "profiled plain", profile=True, prof=prof,
-                       static_params=static)
+    plain_prof = dmf.default_params(g, variant=args.variant, mode=1, relabel_interval=args.interval)
+    precs, _ = run_gpu(g, s, t, batches, plain_prof, args.roofline_steps, 2, "profiled plain", profile=True,
+                       prof=prof, static_params=dmf.default_params(g, variant=args.static_variant, mode=1))
     roof = roofline(prof)
     if args.roofline_only:
         print(json.dumps({"roofline": roof, "profiled_p50_ms": nearest_rank([r["device_ms"] for r in precs], 50)}),
@@ -317,7 +322,8 @@ def main():
     S = summarize(recs)
     L = None
     if args.sched != "plain":
-        learned = dmf.default_params(g, variant=args.variant, mode=1, scheduler=dmf.SCHED_LEARNED)
+        learned = dmf.default_params(g, variant=args.variant, mode=args.mode if args.mode else 1,
+                                     scheduler=dmf.SCHED_LEARNED)
         lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned",
                                 static_params=static)
         L = summarize(lrecs)
@@ -348,7 +354,7 @@ def main():
             "n": g.n, "m": g.m, "s": s, "t": t,
             "variant": args.variant, "kc": kc, "relabel_interval": args.interval,
             "static_solve_variant": args.static_variant,
-            "driver": "persistent" if args.mode == 0 else "stepped",
+            "driver": {0: "persistent", 1: "stepped", 2: "stepped, BFS levels replayed from hipGraphs"}[args.mode],
         },
         "scheduler": "learned" if L is not None else "fixed",
         "p99_ms": round(head["p99_ms"], 4),

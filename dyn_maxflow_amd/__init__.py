@@ -386,8 +386,10 @@ def device_count() -> int:
     return int(_lib().dmf_device_count())
 
 
-def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=0,
+def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=None,
                    max_rounds=0, scheduler=SCHED_FIXED, lazy_start=0) -> Params:
+    """dmf_params_default, then the given fields (mode None keeps the engine's
+    default driver: stepped with graph-replayed BFS levels)."""
     p = Params()
     c = g._csr()
     _lib().dmf_params_default(C.byref(c), C.byref(p))
@@ -395,7 +397,8 @@ def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=0
     if kc is not None:
         p.kc = kc
     p.relabel_interval = relabel_interval
-    p.mode = mode
+    if mode is not None:
+        p.mode = mode
     p.max_rounds = max_rounds
     p.scheduler = scheduler
     p.lazy_start = lazy_start

@@ -17,8 +17,10 @@
 //                                  applies the files one after another.
 //   --kc=N                         run only this kernel_cycles value
 //   --relabel-interval=N           rounds between global relabels (default 1)
-//   --stepped                      per-phase kernel launches instead of the
-//                                  one-launch persistent solve
+//   --driver=graph|stepped|persistent  engine driver (default graph: per-phase
+//                                  launches with the BFS levels replayed from
+//                                  captured hipGraphs); --stepped and
+//                                  --persistent are short forms
 //   --json                         one JSON line of counters per run
 //   --no-static-check              skip the static recompute
 // Compiled once per variant with -DDMF_BENCH_VARIANT="data|topo|pp" and
@@ -45,7 +47,7 @@ struct Opts {
   bool successive = false;
   int kc_only = 0;
   int interval = 1;
-  bool stepped = false;
+  int driver = 2;  // dmf_params.mode
   bool json = false;
   bool static_check = true;
 };
@@ -103,7 +105,7 @@ int dynamic_run(const dmf_csr& g, const std::vector<int32_t>& src, const std::ve
   p.variant = variant;
   p.kc = kc;
   p.relabel_interval = o.interval;
-  p.mode = o.stepped ? 1 : 0;
+  p.mode = o.driver;
   int64_t f = 0;
   rc = dmf_engine_static_solve(E, &p, &f);
   if (rc) {
@@ -162,7 +164,9 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(x, "--mode=independent")) o.successive = false;
     else if (!std::strncmp(x, "--kc=", 5)) o.kc_only = std::atoi(x + 5);
     else if (!std::strncmp(x, "--relabel-interval=", 19)) o.interval = std::atoi(x + 19);
-    else if (!std::strcmp(x, "--stepped")) o.stepped = true;
+    else if (!std::strcmp(x, "--stepped") || !std::strcmp(x, "--driver=stepped")) o.driver = 1;
+    else if (!std::strcmp(x, "--persistent") || !std::strcmp(x, "--driver=persistent")) o.driver = 0;
+    else if (!std::strcmp(x, "--driver=graph")) o.driver = 2;
     else if (!std::strcmp(x, "--json")) o.json = true;
     else if (!std::strcmp(x, "--no-static-check")) o.static_check = false;
     else {
@@ -192,7 +196,7 @@ int main(int argc, char** argv) {
     dmf_params_default(&g, &rp);
     rp.variant = variant_id();
     rp.relabel_interval = o.interval;
-    rp.mode = o.stepped ? 1 : 0;
+    rp.mode = o.driver;
     if (o.kc_only > 0) rp.kc = o.kc_only;
     int64_t f0 = 0;
     rc = dmf_engine_static_solve(resident, &rp, &f0);

@@ -879,7 +879,7 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
   p->relabel_interval = 1;
   p->scheduler = DMF_SCHED_FIXED;
   p->max_rounds = 0;
-  p->mode = 1;  // stepped: the fastest driver on R-MAT graphs
+  p->mode = 2;  // stepped with graph-replayed BFS levels: the fastest driver
   p->lazy_start = 0;
 }
 

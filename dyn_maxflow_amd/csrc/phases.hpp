@@ -1489,11 +1489,18 @@ __global__ void k_store_dev(Dev D, Dev* out) {
 // which no block of this launch reads, and clears ring slot (lvl+2)%3 (the
 // size of level lvl-1, already folded into the cursor; level lvl+1 appends
 // to it).
+// The cursor, the frontier size and the wide-queue word of the current level
+// are read with ordinary (scalar, cached) loads: none of them is written
+// during this launch, and the previous launch's writes are visible at the
+// launch boundary.  Uncached loads of the same words by every wave of a large
+// grid would serialise at one memory channel.
 __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, int mode, int r) {
   const Dev D = *pD;
-  const int lo = ldr(&D.ctrl->bcur[r][0]);
-  const int lvl = ldr(&D.ctrl->bcur[r][1]);
-  const int cnt = ldr(&D.ctrl->qcnt[lvl % 3]);
+  const Ctrl* cc = D.ctrl;
+  const int lo = cc->bcur[r][0];
+  const int lvl = cc->bcur[r][1];
+  const int cnt = cc->qcnt[lvl % 3];
+  const unsigned long long wpk = cc->wq[lvl % 3];
   if (is_block0_thread0()) {
     D.ctrl->bcur[r ^ 1][0] = lo + cnt;
     D.ctrl->bcur[r ^ 1][1] = cnt ? lvl + 1 : lvl;
@@ -1504,7 +1511,6 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
   }
   if (cnt == 0) return;  // uniform: the BFS is over
   Tally t;
-  const unsigned long long wpk = ldru64(&D.ctrl->wq[lvl % 3]);
   if (use_bottom_up(D, mode, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
     ph_bfs_bottomup(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
   } else {

@@ -147,7 +147,7 @@ int main(int argc, char** argv) {
   dmf_server_config cfg;
   dmf_server_config_default(&cfg);
   cfg.params.scheduler = learned ? DMF_SCHED_LEARNED : DMF_SCHED_FIXED;
-  cfg.params.mode = 1;
+  cfg.params.mode = 2;
   cfg.staging_bytes = std::max<int64_t>(int64_t{64} << 20, kbig * 12 * 4);  // room for 4 of the largest
   dmf_server* S = nullptr;
   if ((rc = dmf_server_create(&cfg, &S))) die("server_create", rc);

@@ -244,7 +244,7 @@ void dmf_server_config_default(dmf_server_config* c) {
   c->params.relabel_interval = 1;
   c->params.scheduler = DMF_SCHED_LEARNED;
   c->params.max_rounds = 0;
-  c->params.mode = 1;
+  c->params.mode = 2;
 }
 
 int dmf_server_create(const dmf_server_config* cfg, dmf_server** out) {

@@ -4,8 +4,8 @@
   python -m dyn_maxflow_amd.sched_train train --traces DIR  (CPU; writes csrc/sched_weights.inc)
 
 collect: for a set of training workloads (R-MAT scale 15-20 with seeds 11-16
-and small grids, several batch sizes and increase/decrease mixes; the RMAT-20
-bench graph, seed 2, is not among them) every batch is solved under every
+and 128/256 grids, several batch sizes and increase/decrease mixes; none uses
+seed 2, the seed of the RMAT-20 bench graph) every batch is solved under every
 fixed schedule in
 {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}, data-driven ones
 also with the lazy start (no relabel before the first round), each from a
@@ -14,8 +14,9 @@ trace line per scheduler evaluation (the 8 device-computed features, the round,
 whether that round started with a global relabel) and one summary line per
 solve (device ms, variant, interval).
 
-train: for every (workload, batch) the fastest schedule is the teacher.  The
-variant head learns that schedule's variant from the batch-start features; the
+train: the fastest schedule of each batch is its teacher (--teacher workload:
+the schedule with the lowest total time over all of a workload's batches).
+The variant head learns that schedule's variant from the batch-start features; the
 relabel head learns whether the teacher relabelled before its first round
 (from the batch-start features; data-driven schedules may start on the labels
 of the previous solve) and, from the features at the end of each round of the
@@ -53,7 +54,7 @@ def _workloads():
     return out
 
 
-def collect(out_dir: str, quick: bool = False) -> None:
+def collect(out_dir: str, quick: bool = False, mode: int = 1) -> None:
     sys.path.insert(0, ROOT)
     import dyn_maxflow_amd as dmf
     os.makedirs(out_dir, exist_ok=True)
@@ -76,13 +77,14 @@ def collect(out_dir: str, quick: bool = False) -> None:
         with dmf.Engine(g, s, t) as eng:
             os.environ.pop("DMF_TRACE", None)
             # push-pull static state: every variant may take the batches
-            base = dmf.default_params(g, variant="pp")
+            base = dmf.default_params(g, variant="pp", mode=mode)
             for variant in VARIANTS:
                 for interval in ([1, 3] if quick else INTERVALS):
                     for lazy in ((0, 1) if variant == "data" else (0,)):
                         eng.reset()
                         eng.static_solve(base)
-                        p = dmf.default_params(g, variant=variant, relabel_interval=interval, lazy_start=lazy)
+                        p = dmf.default_params(g, variant=variant, relabel_interval=interval, lazy_start=lazy,
+                                               mode=mode)
                         for src, dst, nw in batches:
                             eng.apply_batch(src, dst, nw, p)
                         print(f"[collect] {name} {variant} interval={interval} lazy={lazy}", flush=True)
@@ -105,8 +107,12 @@ def _load(trace_dir: str):
     return data
 
 
-def _examples(data):
-    """(X_relabel, y_relabel, X_variant, y_variant) from the teacher schedules."""
+def _examples(data, teacher="batch"):
+    """(X_relabel, y_relabel, X_variant, y_variant) from the teacher schedules.
+    teacher="workload": the schedule with the lowest total time over all of a
+    workload's batches teaches every batch of it (per-batch winners flip
+    between near-equal schedules on timing noise); "batch": the fastest
+    schedule of each batch."""
     Xr, yr, Xv, yv = [], [], [], []
     for fname, solves in data.items():
         # batch solves grouped by schedule, in order; static solves (entry 0) skipped
@@ -120,8 +126,9 @@ def _examples(data):
         if not runs:
             continue
         nb = min(len(r) for r in runs.values())
+        overall = min(runs, key=lambda k: sum(r["summary"]["device_ms"] for r in runs[k][:nb]))
         for b in range(nb):
-            best = min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
+            best = overall if teacher == "workload" else min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
             teacher = runs[best][b]
             batch_rows = [r for r in teacher["rows"] if r["what"] == "batch"]
             rows = [r for r in teacher["rows"] if r["what"] == "round"]
@@ -206,9 +213,11 @@ def write_weights(P, note: str) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
-def train(trace_dir: str) -> None:
-    data = _load(trace_dir)
-    Xr, yr, Xv, yv = _examples(data)
+def train(trace_dirs, teacher="batch") -> None:
+    data = {}
+    for d in trace_dirs:
+        data.update({f"{d}/{k}": v for k, v in _load(d).items()})
+    Xr, yr, Xv, yv = _examples(data, teacher)
     print(f"[train] relabel examples {len(Xr)} (positive {yr.mean() if len(yr) else 0:.2f}), "
           f"variant examples {len(Xv)} (counts {np.bincount(yv, minlength=3).tolist()})")
     P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
@@ -224,13 +233,15 @@ def main():
     c = sub.add_parser("collect")
     c.add_argument("--out", required=True)
     c.add_argument("--quick", action="store_true")
+    c.add_argument("--mode", type=int, default=1, help="driver the traces are taken on (1 stepped, 2 graph)")
     t = sub.add_parser("train")
-    t.add_argument("--traces", required=True)
+    t.add_argument("--traces", required=True, nargs="+")
+    t.add_argument("--teacher", choices=["workload", "batch"], default="batch")
     a = ap.parse_args()
     if a.cmd == "collect":
-        collect(a.out, a.quick)
+        collect(a.out, a.quick, a.mode)
     else:
-        train(a.traces)
+        train(a.traces, a.teacher)
 
 
 if __name__ == "__main__":

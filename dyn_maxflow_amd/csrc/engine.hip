@@ -769,6 +769,7 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
   uint8_t *side, *pfinal;
   Ctrl* ctrl;
   unsigned long long* hkey;
+  uint16_t* lev16;
   int rc = DMF_OK;
   if ((rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
       (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
@@ -779,7 +780,7 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
-      (inst && (rc = dalloc(E, &d_inst, n)))) {
+      (rc = dalloc(E, &lev16, n)) || (inst && (rc = dalloc(E, &d_inst, n)))) {
     dmf_engine_destroy(E);
     return rc;
   }
@@ -788,6 +789,7 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
   D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
   D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
+  D.lev16 = lev16;
   {
     // direction-optimizing relabel: off unless DMF_BFS_BOTTOM_UP gives the
     // divisor (on residual graphs near a maximum flow the source side is
@@ -958,6 +960,7 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(const_cast<int32_t*>(D.snks));
   hipFree(const_cast<int32_t*>(D.inst));
   hipFree(D.iflow);
+  hipFree(D.lev16);
   hipFree(const_cast<uint32_t*>(D.widebits));
   hipFree(D.wq_vert);
   hipFree(D.wq_base);
@@ -1127,6 +1130,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   int64_t b = 4 * (N + 1);                      // row
   b += 7 * 4 * M;                               // col rev cap cap0 res rres owner
   b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst
+  b += 2 * N;                                   // 16-bit BFS levels
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 48 * (M / 64 + 1);                       // wide lists and queues (wide_t >= 64)

@@ -40,6 +40,7 @@ constexpr int kChunk = 256;
 constexpr int kGroup = 16;      // frontier vertices per wave task in the flattened BFS expansion
 constexpr int kSlots = 64;       // work counters are spread over this many addresses
 constexpr int kStageCap = 512;   // per-wave LDS staging of frontier appends
+constexpr uint16_t kLevFree = 0xFFFF;  // lev16 value of an unlabelled vertex
 
 enum CtrIdx {
   C_ARC = 0,
@@ -111,6 +112,7 @@ struct Dev {
   uint8_t* pfinal;
   Ctrl* ctrl;
   unsigned long long* ctrs;  // C_N x kSlots work counters
+  uint16_t* lev16;           // BFS filter: 0xFFFF unlabelled in the current relabel, else min(level, 0xFFFE)
   int32_t bottom_up;         // bottom-up BFS level once frontier arcs > m / bottom_up (0 = never)
   unsigned long long* hkey;
   int32_t* hval;
@@ -468,6 +470,7 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
         root = is_term(D, v) || (inB && ex < 0) || (!inB && ex > 0);
       }
       D.height[v] = root ? 0 : n;
+      D.lev16[v] = root ? 0 : kLevFree;
       t.vert += 1;
       if (root) {
         const int d = degree(D, v);
@@ -490,6 +493,7 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
       v = D.part[i];
       root = D.excess[v] < 0;
       D.height[v] = root ? 0 : D.n;
+      D.lev16[v] = root ? 0 : kLevFree;
       t.vert += 1;
       if (root) {
         const int d = degree(D, v);
@@ -513,10 +517,15 @@ __device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int
 // A vertex is unlabelled while its height is n; a level claims it by
 // compare-and-swap from n (one winner per vertex).  Claims stay on distinct
 // words: a packed one-bit-per-vertex mark array was measured slower, its
-// atomics colliding on the words 32 vertices share.
-__device__ __forceinline__ bool bfs_unlabelled(const Dev& D, int w) { return D.height[w] == D.n; }
+// atomics colliding on the words 32 vertices share.  The test before the
+// claim reads the 16-bit copy lev16 (half the bytes of height[], so more of it
+// stays in cache); the winner writes it.  A stale copy only costs a claim
+// that fails.
+__device__ __forceinline__ bool bfs_unlabelled(const Dev& D, int w) { return D.lev16[w] == kLevFree; }
 __device__ __forceinline__ bool bfs_claim(const Dev& D, int w, int nb) {
-  return atomicCAS(&D.height[w], D.n, nb) == D.n;
+  const bool won = atomicCAS(&D.height[w], D.n, nb) == D.n;
+  if (won) D.lev16[w] = static_cast<uint16_t>(nb < kLevFree ? nb : kLevFree - 1);
+  return won;
 }
 
 // Whether a BFS level may label head w through arc e of frontier vertex v is
@@ -717,6 +726,7 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
     if (hit) {
       // w is owned by this thread alone: a plain store labels it
       D.height[w] = lvl + 1;
+      D.lev16[w] = static_cast<uint16_t>(lvl + 1 < kLevFree ? lvl + 1 : kLevFree - 1);
       if (deg > D.wide_t) wq_append(D, qnext, w, deg);
     }
     st.push(hit, w, next_cnt, out);

@@ -857,35 +857,51 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
     if (!(budget > 0 && (PULL || hv < n))) break;
     long long moved = 0;
     int mo = INT_MAX;
-    for (int e = b; e < b + deg && moved < budget; ++e) {
-      const int w = D.col[e];
-      if (!push_filter(D, pmode, w)) continue;
-      const int resid = PULL ? ldr(&D.rres[e]) : ldr(&D.res[e]);
-      if (resid <= 0) continue;
-      const int hw = D.height[w];
-      if (hw >= hv) {
-        mo = min(mo, hw);
-        continue;
+    const int end = b + deg;
+    // four arcs at a time: their heads, residuals and head heights are loaded
+    // before any of them is used, so one lane keeps several loads in flight
+    for (int e0 = b; e0 < end && moved < budget; e0 += 4) {
+      int wv[4], rv[4], hv4[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) wv[j] = e0 + j < end ? D.col[e0 + j] : -1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool ok = wv[j] >= 0 && push_filter(D, pmode, wv[j]);
+        rv[j] = ok ? (PULL ? ldr(&D.rres[e0 + j]) : ldr(&D.res[e0 + j])) : 0;
       }
-      const long long left = budget - moved;
-      const int d = static_cast<int>(left < resid ? left : resid);
-      const int p = D.rev[e];
-      if (PULL) {
-        atomicSub(&D.rres[e], d);
-        atomicAdd(&D.res[e], d);
-        atomicAdd(&D.rres[p], d);
-        atomicSub(&D.res[p], d);
-        add64(&D.excess[w], -static_cast<long long>(d));
-      } else {
-        atomicSub(&D.res[e], d);
-        atomicAdd(&D.rres[e], d);
-        atomicAdd(&D.res[p], d);
-        atomicSub(&D.rres[p], d);
-        add64(&D.excess[w], d);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hv4[j] = rv[j] > 0 ? D.height[wv[j]] : INT_MAX;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int e = e0 + j;
+        const int w = wv[j];
+        const int resid = rv[j];
+        if (resid <= 0 || moved >= budget) continue;
+        const int hw = hv4[j];
+        if (hw >= hv) {
+          mo = min(mo, hw);
+          continue;
+        }
+        const long long left = budget - moved;
+        const int d = static_cast<int>(left < resid ? left : resid);
+        const int p = D.rev[e];
+        if (PULL) {
+          atomicSub(&D.rres[e], d);
+          atomicAdd(&D.res[e], d);
+          atomicAdd(&D.rres[p], d);
+          atomicSub(&D.res[p], d);
+          add64(&D.excess[w], -static_cast<long long>(d));
+        } else {
+          atomicSub(&D.res[e], d);
+          atomicAdd(&D.rres[e], d);
+          atomicAdd(&D.res[p], d);
+          atomicSub(&D.rres[p], d);
+          add64(&D.excess[w], d);
+        }
+        moved += d;
+        t.push += 1;
+        t.atom += 5;
       }
-      moved += d;
-      t.push += 1;
-      t.atom += 5;
     }
     t.arc += deg;
     if (moved) {
@@ -1108,6 +1124,51 @@ __device__ __forceinline__ void repair_one(const Dev& D, int rmode, int v, int h
   else repair_arc(D, v, h, e, w, t);
 }
 
+// Lane path of the repair: four arcs at a time, heads, residuals and head
+// heights loaded before any of them is tested (one lane keeps several loads in
+// flight); each arc is then repaired exactly as repair_one would.
+__device__ __forceinline__ void repair_lane(const Dev& D, int rmode, int v, int h, int b, int end, Tally& t) {
+  for (int e0 = b; e0 < end; e0 += 4) {
+    int wv[4], dv[4], hw[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) wv[j] = e0 + j < end ? D.col[e0 + j] : -1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int w = wv[j];
+      bool ok = w >= 0;
+      if (ok && rmode == RM_B) ok = D.side[w] != 0;
+      if (ok && rmode == RM_PULL_A) ok = D.side[w] == 0;
+      if (ok && rmode == RM_PFINAL) ok = D.pfinal[w] != 0;
+      dv[j] = ok ? (rmode == RM_PULL_A ? D.res[e0 + j] : D.rres[e0 + j]) : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) hw[j] = dv[j] > 0 ? D.height[wv[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (wv[j] >= 0) t.arc += 1;
+      if (dv[j] <= 0 || hw[j] <= h + 1) continue;
+      const int e = e0 + j, w = wv[j], d = dv[j], p = D.rev[e];
+      if (rmode == RM_PULL_A) {
+        D.rres[e] += d;
+        D.res[e] = 0;
+        D.rres[p] -= d;
+        D.res[p] += d;
+        add64(&D.excess[v], -static_cast<long long>(d));
+        add64(&D.excess[w], d);
+      } else {
+        D.rres[e] = 0;
+        D.res[e] += d;
+        D.rres[p] += d;
+        D.res[p] -= d;
+        add64(&D.excess[w], -static_cast<long long>(d));
+        add64(&D.excess[v], d);
+      }
+      t.repair += 1;
+      t.atom += 2;
+    }
+  }
+}
+
 __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int cnt, int rmode,
                           const WList& W, Tally& t) {
   for (int c = r.wid; c < W.chunks; c += r.nw) {
@@ -1139,8 +1200,7 @@ __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int
       const int bh = __shfl(h, leader);
       for (int e = bb + lane_id(); e < bb + bd; e += kWave) repair_one(D, rmode, bv, bh, e, t);
     }
-    if (v >= 0 && deg <= D.lane_max)
-      for (int e = b; e < b + deg; ++e) repair_one(D, rmode, v, h, e, t);
+    if (v >= 0 && deg <= D.lane_max) repair_lane(D, rmode, v, h, b, b + deg, t);
   }
 }
 

@@ -278,6 +278,41 @@ struct Stage {
     cnt += __popcll(b);
     if (cnt > kStageCap - kWave) flush(gcnt, out);
   }
+  // Final publication by the whole block: one atomic for what all its waves
+  // still hold, instead of one per wave (a large level ends with thousands of
+  // waves flushing, and same-address atomics serialise).  Every thread of the
+  // block must call it.
+  __device__ void flush_block(int32_t* gcnt, int32_t* out) {
+    __shared__ int32_t wcnt[kBlock / kWave];
+    __shared__ int32_t bbase;
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) wcnt[w] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int i = 0; i < kBlock / kWave; ++i) tot += wcnt[i];
+      bbase = tot ? atomicAdd(gcnt, tot) : 0;
+    }
+    __syncthreads();
+    int off = bbase;
+    for (int i = 0; i < w; ++i) off += wcnt[i];
+    for (int i = lane_id(); i < cnt; i += kWave) out[off + i] = buf[i];
+    cnt = 0;
+  }
+};
+
+// Block staging of wide-queue appends: the block's newly labelled wide
+// vertices are collected in LDS and published by wave 0 at the end with one
+// atomic that reserves their slots and chunks together; chunk bases are an
+// exclusive prefix sum in slot order, so base[] stays increasing as
+// wave_find_slot requires.  Past kWideStage entries a vertex is appended
+// directly (wq_append), which reserves its own slot and chunks in one atomic
+// and keeps the same order property.
+constexpr int kWideStage = 256;
+struct WideStage {
+  int32_t* vert;    // LDS, kWideStage
+  int32_t* chunks;  // LDS, kWideStage
+  int32_t* cnt;     // LDS
 };
 
 __device__ __forceinline__ int wave_incl_scan_i32(int x) {
@@ -338,6 +373,56 @@ __device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
   const size_t slot = static_cast<size_t>(old >> 32);
   D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
   D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu);
+}
+
+// Lanes with pred set note wide vertex v (degree deg) into the block stage.
+// All 64 lanes.
+__device__ __forceinline__ void wide_note(const Dev& D, const WideStage& S, int q, bool pred, int v) {
+  const unsigned long long b = __ballot(pred);
+  if (b == 0) return;
+  const int leader = __ffsll(static_cast<long long>(b)) - 1;
+  int base = 0;
+  if (lane_id() == leader) base = atomicAdd(S.cnt, __popcll(b));
+  base = __shfl(base, leader);
+  if (pred) {
+    const int deg = D.row[v + 1] - D.row[v];
+    const int idx = base + __popcll(b & ((1ull << lane_id()) - 1ull));
+    if (idx < kWideStage) {
+      S.vert[idx] = v;
+      S.chunks[idx] = (deg + kChunk - 1) / kChunk;
+    } else {
+      wq_append(D, q, v, deg);
+    }
+  }
+}
+// Publishes the block's staged wide vertices into queue q.  Every thread of
+// the block must call it; the caller has made the stage visible
+// (__syncthreads) since the last wide_note.
+__device__ __forceinline__ void wide_flush(const Dev& D, const WideStage& S, int q) {
+  if (threadIdx.x >= kWave) return;
+  const int nv = min(*S.cnt, kWideStage);
+  if (nv == 0) return;
+  int sum = 0;
+  for (int i = lane_id(); i < nv; i += kWave) sum += S.chunks[i];
+  sum = __shfl(wave_incl_scan_i32(sum), kWave - 1);
+  unsigned long long old = 0;
+  if (lane_id() == 0)
+    old = atomicAdd(&D.ctrl->wq[q], (static_cast<unsigned long long>(nv) << 32) | static_cast<unsigned>(sum));
+  old = __shfl(old, 0);
+  int slot = static_cast<int>(old >> 32);
+  int run = static_cast<int>(old & 0xffffffffu);
+  int32_t* vert = D.wq_vert + static_cast<size_t>(q) * D.wq_cap;
+  int32_t* base = D.wq_base + static_cast<size_t>(q) * D.wq_cap;
+  for (int i0 = 0; i0 < nv; i0 += kWave) {
+    const int i = i0 + lane_id();
+    const int c = i < nv ? S.chunks[i] : 0;
+    const int incl = wave_incl_scan_i32(c);
+    if (i < nv) {
+      vert[slot + i] = S.vert[i];
+      base[slot + i] = run + incl - c;
+    }
+    run += __shfl(incl, kWave - 1);
+  }
 }
 
 // Largest slot a with base[a] <= c, found by the whole wave: each step every
@@ -510,9 +595,6 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
 __device__ __forceinline__ bool is_wide(const Dev& D, int v) {
   return (D.widebits[static_cast<unsigned>(v) >> 5] >> (v & 31)) & 1u;
 }
-__device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int qnext) {
-  if (hit && is_wide(D, w)) wq_append(D, qnext, w, degree(D, w));
-}
 
 // A vertex is unlabelled while its height is n; a level claims it by
 // compare-and-swap from n (one winner per vertex).  Claims stay on distinct
@@ -520,13 +602,8 @@ __device__ __forceinline__ void bfs_note_wide(const Dev& D, bool hit, int w, int
 // atomics colliding on the words 32 vertices share.  The test before the
 // claim reads the 16-bit copy lev16 (half the bytes of height[], so more of it
 // stays in cache); the winner writes it.  A stale copy only costs a claim
-// that fails.
+// that fails.  (The claim itself is issued in ph_bfs_level's step.)
 __device__ __forceinline__ bool bfs_unlabelled(const Dev& D, int w) { return D.lev16[w] == kLevFree; }
-__device__ __forceinline__ bool bfs_claim(const Dev& D, int w, int nb) {
-  const bool won = atomicCAS(&D.height[w], D.n, nb) == D.n;
-  if (won) D.lev16[w] = static_cast<uint16_t>(nb < kLevFree ? nb : kLevFree - 1);
-  return won;
-}
 
 // Whether a BFS level may label head w through arc e of frontier vertex v is
 // split in two, so that the residual can be loaded a tile ahead of its use:
@@ -550,43 +627,60 @@ __device__ __forceinline__ bool bfs_head_ok(const Dev& D, int mode, int w) {
   return !D.side[w] && D.pfinal[w];
 }
 
-// One pair of 64-arc tiles with their heads and residuals loaded.
+// kBfsTiles 64-arc tiles with their heads and residuals loaded: one step of a
+// wave's BFS expansion.
+constexpr int kBfsTiles = 2;
 struct BfsTiles {
-  int wA, wB, rA, rB;
-  bool inA, inB;
+  int w[kBfsTiles], r[kBfsTiles];
+  bool in[kBfsTiles];
 };
 
 // Top-down BFS level (staticMaxFlow_kernel_11, code/dyn_data.h:72-87; pp_bfs,
 // code/dyn_pp.h:99-120; final_bfs, code/dyn_pp.h:123-140).  Frontier is
 // order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
 // get height lvl+1 (one winner per vertex) and are appended after the frontier
-// through per-wave LDS staging.  The 16 narrow frontier vertices a wave takes
-// at a time are expanded as one flattened arc range, so consecutive lanes read
-// consecutive arcs (coalesced col/residual loads) and each lane finds its
-// arc's tail by a 4-step binary search over the group's degree prefix sums.
-// Both loops work on pairs of 64-arc tiles and load the next pair's heads and
-// residuals before testing and claiming the current pair's, so the streaming
-// loads overlap the height gathers and the claims.
+// through per-wave LDS staging, published per block at the end (wide ones
+// through the block's wide stage).  The 16 narrow frontier vertices a wave
+// takes at a time are expanded as one flattened arc range, so consecutive
+// lanes read consecutive arcs (coalesced col/residual loads) and each lane
+// finds its arc's tail by a 4-step binary search over the group's degree
+// prefix sums.  A step covers kBfsTiles tiles: the next step's heads and
+// residuals are loaded before the current step's labels are tested, and all
+// of a step's claims are issued before any of their results is waited for,
+// so each wave keeps several dependent round trips' worth of arcs in flight.
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
                              int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
   Stage st{sbuf[threadIdx.x / kWave], 0};
+  const WideStage WS{wvert, wchunks, &wcnt};
+  if (threadIdx.x == 0) wcnt = 0;
+  __syncthreads();
   const int nb = lvl + 1;
+  const uint16_t nb16 = static_cast<uint16_t>(nb < kLevFree ? nb : kLevFree - 1);
   int32_t* out = D.order + lo + cnt;
   auto expand = [&](const BfsTiles& T) {
-    bool cA = T.inA && T.rA > 0 && bfs_head_ok(D, mode, T.wA);
-    bool cB = T.inB && T.rB > 0 && bfs_head_ok(D, mode, T.wB);
-    cA = cA && bfs_unlabelled(D, T.wA);
-    cB = cB && bfs_unlabelled(D, T.wB);
-    t.arc += (T.inA ? 1 : 0) + (T.inB ? 1 : 0);
-    t.atom += (cA ? 1 : 0) + (cB ? 1 : 0);
-    const bool hitA = cA && bfs_claim(D, T.wA, nb);
-    const bool hitB = cB && bfs_claim(D, T.wB, nb);
-    bfs_note_wide(D, hitA, T.wA, qnext);
-    bfs_note_wide(D, hitB, T.wB, qnext);
-    st.push(hitA, T.wA, next_cnt, out);
-    st.push(hitB, T.wB, next_cnt, out);
+    bool c[kBfsTiles];
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) c[j] = T.in[j] && T.r[j] > 0 && bfs_head_ok(D, mode, T.w[j]);
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) c[j] = c[j] && bfs_unlabelled(D, T.w[j]);
+    int old[kBfsTiles];
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) {
+      t.arc += T.in[j] ? 1 : 0;
+      t.atom += c[j] ? 1 : 0;
+      old[j] = c[j] ? atomicCAS(&D.height[T.w[j]], D.n, nb) : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) {
+      const bool hit = old[j] == D.n;
+      if (hit) D.lev16[T.w[j]] = nb16;
+      wide_note(D, WS, qnext, hit && is_wide(D, T.w[j]), T.w[j]);
+      st.push(hit, T.w[j], next_cnt, out);
+    }
   };
+  constexpr int kStep = kBfsTiles * kWave;
   // wide frontier vertices: chunks spread over every wave of the grid
   for (int c = r.wid; c < W.chunks; c += r.nw) {
     int a, b;
@@ -594,19 +688,19 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     const bool vB = mode == BFS_PP && D.side[v] != 0;
     auto fetch = [&](int j0) {
       BfsTiles T;
-      const int eA = j0 + lane_id(), eB = eA + kWave;
-      T.inA = eA < b;
-      T.inB = eB < b;
-      T.wA = T.inA ? D.col[eA] : 0;
-      T.wB = T.inB ? D.col[eB] : 0;
-      T.rA = T.inA ? bfs_resid(D, mode, vB, eA) : 0;
-      T.rB = T.inB ? bfs_resid(D, mode, vB, eB) : 0;
+#pragma unroll
+      for (int j = 0; j < kBfsTiles; ++j) {
+        const int e = j0 + j * kWave + lane_id();
+        T.in[j] = e < b;
+        T.w[j] = T.in[j] ? D.col[e] : 0;
+        T.r[j] = T.in[j] ? bfs_resid(D, mode, vB, e) : 0;
+      }
       return T;
     };
     BfsTiles cur = fetch(a);
-    for (int j0 = a; j0 < b; j0 += 2 * kWave) {
+    for (int j0 = a; j0 < b; j0 += kStep) {
       const BfsTiles now = cur;
-      if (j0 + 2 * kWave < b) cur = fetch(j0 + 2 * kWave);
+      if (j0 + kStep < b) cur = fetch(j0 + kStep);
       expand(now);
     }
   }
@@ -627,36 +721,33 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     const int excl = incl - deg;
     // called by the whole wave (shuffles) with a wave-uniform k0
     auto fetch = [&](int k0) {
-      const int pA = k0 + lane_id(), pB = pA + kWave;
-      int posA = 0, posB = 0;  // owner lanes: number of group lanes whose prefix end <= p
-#pragma unroll
-      for (int sh = kGroup / 2; sh > 0; sh >>= 1) {
-        if (__shfl(incl, posA + sh - 1) <= pA) posA += sh;
-        if (__shfl(incl, posB + sh - 1) <= pB) posB += sh;
-      }
-      posA &= kGroup - 1;
-      posB &= kGroup - 1;
-      const int eA = __shfl(b, posA) + (pA - __shfl(excl, posA));
-      const int eB = __shfl(b, posB) + (pB - __shfl(excl, posB));
-      const bool okA = __shfl(vB, posA) != 0, okB = __shfl(vB, posB) != 0;
       BfsTiles T;
-      T.inA = pA < total;
-      T.inB = pB < total;
-      T.wA = T.inA ? D.col[eA] : 0;
-      T.wB = T.inB ? D.col[eB] : 0;
-      T.rA = T.inA ? bfs_resid(D, mode, okA, eA) : 0;
-      T.rB = T.inB ? bfs_resid(D, mode, okB, eB) : 0;
+#pragma unroll
+      for (int j = 0; j < kBfsTiles; ++j) {
+        const int p = k0 + j * kWave + lane_id();
+        int pos = 0;  // owner lane: number of group lanes whose prefix end <= p
+#pragma unroll
+        for (int sh = kGroup / 2; sh > 0; sh >>= 1)
+          if (__shfl(incl, pos + sh - 1) <= p) pos += sh;
+        pos &= kGroup - 1;
+        const int e = __shfl(b, pos) + (p - __shfl(excl, pos));
+        const bool ok = __shfl(vB, pos) != 0;
+        T.in[j] = p < total;
+        T.w[j] = T.in[j] ? D.col[e] : 0;
+        T.r[j] = T.in[j] ? bfs_resid(D, mode, ok, e) : 0;
+      }
       return T;
     };
     if (total == 0) continue;
     BfsTiles cur = fetch(0);
-    for (int k0 = 0; k0 < total; k0 += 2 * kWave) {
+    for (int k0 = 0; k0 < total; k0 += kStep) {
       const BfsTiles now = cur;
-      if (k0 + 2 * kWave < total) cur = fetch(k0 + 2 * kWave);
+      if (k0 + kStep < total) cur = fetch(k0 + kStep);
       expand(now);
     }
   }
-  st.flush(next_cnt, out);
+  st.flush_block(next_cnt, out);  // its barriers also publish the wide stage
+  wide_flush(D, WS, qnext);
 }
 
 // Bottom-up BFS level over the reverse residual graph (same levels as the

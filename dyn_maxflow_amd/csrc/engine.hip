@@ -60,6 +60,11 @@ struct dmf_engine {
   int32_t ucap = 0;
   int coop_blocks = 0;
   int num_cu = 0;
+  // Most blocks one stepped launch uses: a few per CU, each wave looping over
+  // its share (every kernel is grid-stride).  A grid of thousands of
+  // short-lived blocks is limited by how fast blocks are dispatched and
+  // retired, not by the work in them.
+  int grid_cap = 8192;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   dmf_counters last{};
   int64_t bytes = 0;
@@ -124,6 +129,8 @@ int dalloc(dmf_engine* E, T** p, size_t count) {
   return DMF_OK;
 }
 
+constexpr int kGridBlocksPerCu = 4;
+
 int blocks_for(int64_t work) {
   int64_t b = (work + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -145,7 +152,8 @@ int read_int(dmf_engine* E, const int32_t* dptr, int32_t* out) {
 
 #define LAUNCH(kernel, blocks, ...)                                                   \
   do {                                                                                \
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kBlock), 0, E->stream, __VA_ARGS__); \
+    hipLaunchKernelGGL(kernel, dim3(std::min<int>((blocks), E->grid_cap)), dim3(kBlock), 0, E->stream, \
+                       __VA_ARGS__);                                                  \
     ++E->launches;                                                                    \
     HIP_OK(hipGetLastError());                                                        \
   } while (0)
@@ -191,7 +199,7 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
     const unsigned long long wpk = E->h_ctrl->wq[lvl % 3];
     if (cnt == 0) break;
     const int bu = use_bottom_up(D, mode, cnt, chunks_of(wpk)) ? 1 : 0;
-    const int grid = bu ? blocks_for(E->n) : bfs_blocks(cnt, chunks_of(wpk));
+    const int grid = std::min(bu ? blocks_for(E->n) : bfs_blocks(cnt, chunks_of(wpk)), E->grid_cap);
     if (E->profile) {
       // events stamped by the launch itself: the span is the kernel's execution
       hipExtLaunchKernelGGL(k_bfs_level, dim3(grid), dim3(kBlock), 0, E->stream, E->pev0, E->pev1, 0, D, mode, lo,
@@ -220,7 +228,7 @@ int capture_segment(dmf_engine* E, int mode, int idx) {
   if (E->seg_exec[mode][idx]) return DMF_OK;
   // fixed at capture: at most 8 blocks per CU (a full grid makes every empty
   // trailing level of a segment expensive)
-  const int grid = std::min(phase_blocks(E->n, chunks_of(E->D.wide_packed)), std::max(1, E->num_cu) * 8);
+  const int grid = std::min(phase_blocks(E->n, chunks_of(E->D.wide_packed)), E->grid_cap);
   hipStream_t st = E->stream;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
   bool ok = true;
@@ -358,7 +366,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   const bool learned = C.scheduler == DMF_SCHED_LEARNED;
   const bool consult = learned || E->trace != nullptr;
   const int n = E->n;
-  const int64_t static_chunks = chunks_of(D.wide_packed);
+  const int64_t static_chunks = chunks_of(D.wave_packed);  // topology-driven push and repair
   if (C.entry == 0) {
     LAUNCH(k_init_residual, blocks_for(std::max(E->m, E->n)), D);
     LAUNCH(k_saturate_source, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
@@ -384,7 +392,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   if (C.entry == 1) {
     LAUNCH(k_upd_apply, blocks_for(std::max<int64_t>(D.k, D.wide_packed >> 32)), D, 0, pp ? 1 : 0);
     if (pp) {
-      LAUNCH(k_pp_cut, phase_blocks(n, static_chunks), D);
+      LAUNCH(k_pp_cut, phase_blocks(n, chunks_of(D.wide_packed)), D);
     } else {
       LAUNCH(k_upd_clamp, blocks_for(D.k), D);
       LAUNCH(k_saturate_source, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
@@ -686,10 +694,19 @@ namespace {
 // chunk queues sized for it.
 int setup_wide(dmf_engine* E, int32_t wide_t) {
   Dev& D = E->D;
-  std::vector<int32_t> wide_v, wide_b;
-  int64_t wide_chunks = 0;
+  std::vector<int32_t> wide_v, wide_b, wave_v, wave_b;
+  int64_t wide_chunks = 0, wave_chunks = 0;
+  // Vertices above lane_max: push and repair give each its own wave (one per
+  // chunk), from the active queues or, topology-driven, from the static wave
+  // list; the queues have a slot for every such vertex.
+  const int32_t wave_t = std::min(D.lane_max, wide_t);
   for (int32_t v = 0; v < E->n; ++v) {
     const int32_t d = E->h_row[v + 1] - E->h_row[v];
+    if (d > wave_t) {
+      wave_v.push_back(v);
+      wave_b.push_back(static_cast<int32_t>(wave_chunks));
+      wave_chunks += (d + kChunk - 1) / kChunk;
+    }
     if (d > wide_t) {
       wide_v.push_back(v);
       wide_b.push_back(static_cast<int32_t>(wide_chunks));
@@ -697,21 +714,26 @@ int setup_wide(dmf_engine* E, int32_t wide_t) {
     }
   }
   const size_t nwide = std::max<size_t>(1, wide_v.size());
+  const size_t nqueue = std::max<size_t>(1, wave_v.size());
   std::vector<uint32_t> wbits((static_cast<size_t>(E->n) + 31) / 32 + 1, 0u);
   for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
   HIP_OK(hipStreamSynchronize(E->stream));
   for (void* p : {static_cast<void*>(D.wq_vert), static_cast<void*>(D.wq_base),
                   const_cast<void*>(static_cast<const void*>(D.wide_vert)),
                   const_cast<void*>(static_cast<const void*>(D.wide_base)),
+                  const_cast<void*>(static_cast<const void*>(D.wave_vert)),
+                  const_cast<void*>(static_cast<const void*>(D.wave_base)),
                   const_cast<void*>(static_cast<const void*>(D.widebits))})
     if (p) HIP_OK(hipFree(p));
   E->bytes -= E->wide_bytes;
   int32_t *wq_vert = nullptr, *wq_base = nullptr, *wide_vert = nullptr, *wide_base = nullptr;
+  int32_t *wave_vert = nullptr, *wave_base = nullptr;
   uint32_t* widebits = nullptr;
   const int64_t before = E->bytes;
   int rc;
-  if ((rc = dalloc(E, &wq_vert, nwide * kNumQ)) || (rc = dalloc(E, &wq_base, nwide * kNumQ)) ||
+  if ((rc = dalloc(E, &wq_vert, nqueue * kNumQ)) || (rc = dalloc(E, &wq_base, nqueue * kNumQ)) ||
       (rc = dalloc(E, &wide_vert, nwide)) || (rc = dalloc(E, &wide_base, nwide)) ||
+      (rc = dalloc(E, &wave_vert, nqueue)) || (rc = dalloc(E, &wave_base, nqueue)) ||
       (rc = dalloc(E, &widebits, wbits.size())))
     return rc;
   E->wide_bytes = E->bytes - before;
@@ -719,16 +741,24 @@ int setup_wide(dmf_engine* E, int32_t wide_t) {
     HIP_OK(hipMemcpy(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size(), hipMemcpyHostToDevice));
   }
+  if (!wave_v.empty()) {
+    HIP_OK(hipMemcpy(wave_vert, wave_v.data(), sizeof(int32_t) * wave_v.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(wave_base, wave_b.data(), sizeof(int32_t) * wave_b.size(), hipMemcpyHostToDevice));
+  }
   HIP_OK(hipMemcpy(widebits, wbits.data(), sizeof(uint32_t) * wbits.size(), hipMemcpyHostToDevice));
   D.wq_vert = wq_vert;
   D.wq_base = wq_base;
-  D.wq_cap = static_cast<int32_t>(nwide);
+  D.wq_cap = static_cast<int32_t>(nqueue);
   D.wide_vert = wide_vert;
   D.wide_base = wide_base;
   D.widebits = widebits;
   D.wide_t = wide_t;
   D.wide_packed = (static_cast<unsigned long long>(wide_v.size()) << 32) |
                   static_cast<unsigned long long>(wide_chunks);
+  D.wave_vert = wave_vert;
+  D.wave_base = wave_base;
+  D.wave_packed = (static_cast<unsigned long long>(wave_v.size()) << 32) |
+                  static_cast<unsigned long long>(wave_chunks);
   return DMF_OK;
 }
 
@@ -836,18 +866,23 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
     dmf_engine_destroy(E);
     return DMF_EDEVICE;
   }
+  // Stepped grids: kGridBlocksPerCu blocks per CU (DMF_GRID_BLOCKS_PER_CU).
   // Cooperative grid: as many blocks as can be co-resident.
   int dev = 0, per_cu = 0;
   hipDeviceProp_t prop{};
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-      prop.cooperativeLaunch &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_persistent),
-                                                   kBlock, 0) == hipSuccess &&
-      per_cu > 0) {
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
     E->num_cu = prop.multiProcessorCount;
-    const char* capenv = std::getenv("DMF_COOP_BLOCKS_PER_CU");
-    const int want = capenv ? std::max(1, std::atoi(capenv)) : 8;
-    E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
+    const char* genv = std::getenv("DMF_GRID_BLOCKS_PER_CU");
+    const int g = genv ? std::max(1, std::atoi(genv)) : kGridBlocksPerCu;
+    E->grid_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E->num_cu)) * g));
+    if (prop.cooperativeLaunch &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_persistent),
+                                                     kBlock, 0) == hipSuccess &&
+        per_cu > 0) {
+      const char* capenv = std::getenv("DMF_COOP_BLOCKS_PER_CU");
+      const int want = capenv ? std::max(1, std::atoi(capenv)) : 8;
+      E->coop_blocks = prop.multiProcessorCount * std::min(per_cu, want);
+    }
   }
   // scheduler weights (compiled in) and state
   if (hipMalloc(reinterpret_cast<void**>(&E->d_sched_w), sizeof(SchedWeights)) != hipSuccess ||
@@ -966,6 +1001,8 @@ int dmf_engine_destroy(dmf_engine* E) {
   hipFree(D.wq_base);
   hipFree(const_cast<int32_t*>(D.wide_vert));
   hipFree(const_cast<int32_t*>(D.wide_base));
+  hipFree(const_cast<int32_t*>(D.wave_vert));
+  hipFree(const_cast<int32_t*>(D.wave_base));
   hipFree(E->d_usrc);
   hipFree(E->d_udst);
   hipFree(E->d_uw);
@@ -1133,7 +1170,8 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += 2 * N;                                   // 16-bit BFS levels
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
-  b += 48 * (M / 64 + 1);                       // wide lists and queues (wide_t >= 64)
+  b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)
+  b += 48 * std::min(N + 1, M / 17 + 1);        // queues and wave list: per vertex above lane_max (>= 16)
   b += static_cast<int64_t>(C_N) * kSlots * 8;  // counters
   b += int64_t{1} << 20;                        // control block, scheduler, allocation slack
   return b;
@@ -1192,7 +1230,7 @@ int dmf_batch_apply(dmf_engine* E, const int32_t* counts, const int32_t* src, co
 
 int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, int32_t* wide_t) {
   if (!E) return DMF_EINVAL;
-  static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {8, 64}};
+  static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {16, 64}};
   double best_ms = 1e300;
   int best = 1;
   for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {

@@ -34,8 +34,8 @@ namespace dmfdev {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kLaneMax = 32;   // default of Dev::lane_max (thresholds are tunable per engine)
-constexpr int kWideT = 256;    // default of Dev::wide_t
+constexpr int kLaneMax = 16;   // default of Dev::lane_max (thresholds are tunable per engine)
+constexpr int kWideT = 64;     // default of Dev::wide_t
 constexpr int kChunk = 256;
 constexpr int kGroup = 16;      // frontier vertices per wave task in the flattened BFS expansion
 constexpr int kSlots = 64;       // work counters are spread over this many addresses
@@ -125,6 +125,11 @@ struct Dev {
   const int32_t* wide_vert;
   const int32_t* wide_base;
   unsigned long long wide_packed;
+  // static list of every vertex above lane_max (chunked): the topology-driven
+  // push and repair give each its own wave
+  const int32_t* wave_vert;
+  const int32_t* wave_base;
+  unsigned long long wave_packed;
   const uint32_t* widebits;  // bit v set when degree(v) > wide_t
   // current update batch
   const int32_t* usrc;
@@ -358,6 +363,14 @@ __device__ __forceinline__ WList wlist(const Dev& D, int q, unsigned long long p
   L.chunks = static_cast<int>(packed & 0xffffffffu);
   return L;
 }
+__device__ __forceinline__ WList wlist_waves(const Dev& D) {
+  WList L;
+  L.vert = D.wave_vert;
+  L.base = D.wave_base;
+  L.slots = static_cast<int>(D.wave_packed >> 32);
+  L.chunks = static_cast<int>(D.wave_packed & 0xffffffffu);
+  return L;
+}
 __device__ __forceinline__ WList wlist_static(const Dev& D) {
   WList L;
   L.vert = D.wide_vert;
@@ -373,6 +386,27 @@ __device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
   const size_t slot = static_cast<size_t>(old >> 32);
   D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
   D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu);
+}
+
+// Lanes with pred set append vertex v (degree deg) to wide queue q with one
+// atomic for the whole wave: slots by ballot rank, chunk bases by an exclusive
+// prefix sum of the lanes' chunk counts (increasing with the slot, as
+// wave_find_slot requires).  All 64 lanes.
+__device__ __forceinline__ void wave_wq_append(const Dev& D, int q, bool pred, int v, int deg) {
+  const unsigned long long b = __ballot(pred);
+  if (b == 0) return;
+  const int c = pred ? (deg + kChunk - 1) / kChunk : 0;
+  const int incl = wave_incl_scan_i32(c);
+  const int tot = __shfl(incl, kWave - 1);
+  unsigned long long old = 0;
+  if (lane_id() == 0)
+    old = atomicAdd(&D.ctrl->wq[q], (static_cast<unsigned long long>(__popcll(b)) << 32) | static_cast<unsigned>(tot));
+  old = __shfl(old, 0);
+  if (pred) {
+    const size_t slot = static_cast<size_t>(old >> 32) + __popcll(b & ((1ull << lane_id()) - 1ull));
+    D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
+    D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu) + incl - c;
+  }
 }
 
 // Lanes with pred set note wide vertex v (degree deg) into the block stage.
@@ -446,13 +480,16 @@ __device__ __forceinline__ int wave_find_slot(const int32_t* base, int slots, in
   return lo + (63 - __clzll(static_cast<long long>(b)));
 }
 
-// Chunk c of a wide list -> vertex and arc range [lo, hi).  All 64 lanes.
-__device__ __forceinline__ int chunk_range(const Dev& D, const WList& L, int c, int* lo, int* hi) {
+// Chunk c of a wide list -> vertex and arc range [lo, hi); *whole is set when
+// the chunk is the vertex's entire adjacency.  All 64 lanes.
+__device__ __forceinline__ int chunk_range(const Dev& D, const WList& L, int c, int* lo, int* hi,
+                                           bool* whole = nullptr) {
   const int a = wave_find_slot(L.base, L.slots, c);
   const int v = L.vert[a];
-  const int beg = D.row[v];
+  const int beg = D.row[v], end = D.row[v + 1];
   *lo = beg + (c - L.base[a]) * kChunk;
-  *hi = min(*lo + kChunk, D.row[v + 1]);
+  *hi = min(*lo + kChunk, end);
+  if (whole) *whole = end - beg <= kChunk;
   return v;
 }
 
@@ -843,23 +880,26 @@ __host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, i
 // create_wl (code/dyn_data.h:217-238) over the labelled vertices order[0,cnt):
 // excess > 0, v not s or t, height < n.  Narrow and mid-degree vertices go to
 // wl, wide ones to wide queue kQActive.
+// Vertices above lane_max go to the chunked queues, one wave per chunk: a
+// vertex that fits one chunk is discharged by its own wave (kc cycles with
+// local relabels), a larger one chunk by chunk.  Those up to lane_max are
+// listed for one lane each.
 __device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
   for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
     const int i = base + lane_id();
-    bool act = false;
-    int v = -1;
+    bool act = false, wide = false;
+    int v = -1, d = 0;
     if (i < cnt) {
       v = D.order[i];
       act = !is_term(D, v) && D.excess[v] > 0 && D.height[v] < D.n;
       t.vert += 1;
       if (act) {
-        const int d = degree(D, v);
-        if (d > D.wide_t) {
-          wq_append(D, kQActive, v, d);
-          act = false;
-        }
+        d = degree(D, v);
+        wide = d > D.lane_max;
+        act = !wide;
       }
     }
+    wave_wq_append(D, kQActive, wide, v, d);
     int rank;
     const int off = wave_reserve(&D.ctrl->wl_cnt, act, &rank);
     if (act) D.wl[off + rank] = v;
@@ -867,12 +907,13 @@ __device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
 }
 
 // create_pp_wl (code/dyn_pp.h:499-545): B vertices with excess > 0 to wl,
-// A vertices with excess < 0 to wl2 (wide ones to their wide queues).
+// A vertices with excess < 0 to wl2 (those above lane_max to their chunked
+// queues).
 __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
   for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
     const int i = base + lane_id();
-    bool a = false, b = false;
-    int v = -1;
+    bool a = false, b = false, wa = false, wb = false;
+    int v = -1, d = 0;
     if (i < cnt) {
       v = D.order[i];
       if (!is_term(D, v) && D.height[v] < D.n) {
@@ -882,13 +923,16 @@ __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) 
       }
       t.vert += 1;
       if (a || b) {
-        const int d = degree(D, v);
-        if (d > D.wide_t) {
-          wq_append(D, a ? kQActive : kQPull, v, d);
+        d = degree(D, v);
+        if (d > D.lane_max) {
+          wa = a;
+          wb = b;
           a = b = false;
         }
       }
     }
+    wave_wq_append(D, kQActive, wa, v, d);
+    wave_wq_append(D, kQPull, wb, v, d);
     int rank;
     int off = wave_reserve(&D.ctrl->wl_cnt, a, &rank);
     if (a) D.wl[off + rank] = v;
@@ -901,20 +945,19 @@ __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) 
 __device__ void ph_build_final_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
   for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
     const int i = base + lane_id();
-    bool act = false;
-    int v = -1;
+    bool act = false, wide = false;
+    int v = -1, d = 0;
     if (i < cnt) {
       v = D.part[i];
       act = D.excess[v] > 0 && D.height[v] < D.n;
       t.vert += 1;
       if (act) {
-        const int d = degree(D, v);
-        if (d > D.wide_t) {
-          wq_append(D, kQActive, v, d);
-          act = false;
-        }
+        d = degree(D, v);
+        wide = d > D.lane_max;
+        act = !wide;
       }
     }
+    wave_wq_append(D, kQActive, wide, v, d);
     int rank;
     const int off = wave_reserve(&D.ctrl->wl_cnt, act, &rank);
     if (act) D.wl[off + rank] = v;
@@ -1118,14 +1161,38 @@ template <bool PULL>
 __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int cnt, int kc, int pmode,
                         int32_t* flag, const WList& W, Tally& t) {
   const int n = D.n;
+  // Wide chunks: the excess read here only filters out inactive vertices (the
+  // claim in wave_discharge is the exact test) and a wide vertex's height does
+  // not change during a push phase, so both are plain, cached loads.  Uncached
+  // reads of one hub's words by each of its chunks' waves would serialise at
+  // one memory channel.
   for (int c = r.wid; c < W.chunks; c += r.nw) {
     int a, b;
-    const int v = chunk_range(D, W, c, &a, &b);
+    bool whole;
+    const int v = chunk_range(D, W, c, &a, &b, &whole);
+    if (whole) {
+      // a vertex in one chunk: this wave alone discharges it, with kc cycles
+      // and local relabels
+      if (!list) {  // topology-driven: skip inactive vertices, report active ones
+        long long ex0 = 0;
+        int h0 = 0;
+        if (lane_id() == 0) {
+          ex0 = D.excess[v];
+          h0 = D.height[v];
+        }
+        ex0 = __shfl(ex0, 0);
+        h0 = __shfl(h0, 0);
+        if (is_term(D, v) || (PULL ? -ex0 : ex0) <= 0 || (!PULL && h0 >= n)) continue;
+        if (lane_id() == 0 && *flag == 0) *flag = 1;
+      }
+      wave_push_cycles<PULL>(D, v, a, b - a, kc, pmode, t);
+      continue;
+    }
     long long ex = 0;
     int hv = 0;
     if (lane_id() == 0) {
-      ex = ldr64(&D.excess[v]);
-      hv = ldr(&D.height[v]);
+      ex = D.excess[v];
+      hv = D.height[v];
     }
     ex = __shfl(ex, 0);
     hv = __shfl(hv, 0);
@@ -1144,118 +1211,101 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
       v = list ? list[i] : i;
       b = D.row[v];
       deg = D.row[v + 1] - b;
-      if (!list) {
-        const bool act = deg <= D.wide_t && !is_term(D, v) && D.excess[v] > 0 && D.height[v] < n;
+      if (!list) {  // vertices above lane_max come through W
+        const bool act = deg <= D.lane_max && !is_term(D, v) && D.excess[v] > 0 && D.height[v] < n;
         if (!act) v = -1;
         else if (*flag == 0) *flag = 1;
       }
       if (v >= 0) t.vert += 1;
     }
-    unsigned long long big = __ballot(v >= 0 && deg > D.lane_max);
-    while (big) {
-      const int leader = __ffsll(static_cast<long long>(big)) - 1;
-      big &= big - 1;
-      const int bv = __shfl(v, leader);
-      const int bb = __shfl(b, leader);
-      const int bd = __shfl(deg, leader);
-      wave_push_cycles<PULL>(D, bv, bb, bd, kc, pmode, t);
-    }
-    if (v >= 0 && deg <= D.lane_max) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
+    if (v >= 0) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
   }
 }
 
 // --------------------------------------------------------------- repair
-// Label repair on one arc (staticMaxFlow_kernel_17, code/dyn_data.h:140-150):
-// residual w->v exists but h(w) > h(v)+1, so saturate w->v.  Heights are not
-// written during this phase, so at most one endpoint of an arc pair acts.
-__device__ __forceinline__ void repair_arc(const Dev& D, int v, int h, int e, int w, Tally& t) {
-  const int d = D.rres[e];
-  if (d > 0 && D.height[w] > h + 1) {
-    D.rres[e] = 0;
-    D.res[e] += d;
-    const int p = D.rev[e];
-    D.rres[p] += d;
-    D.res[p] -= d;
-    add64(&D.excess[w], -static_cast<long long>(d));
-    add64(&D.excess[v], d);
-    t.repair += 1;
-    t.atom += 2;
-  }
-}
-
-// remove_pull (code/dyn_pp.h:381-403): residual v->w exists but h(w) > h(v)+1,
-// so saturate v->w.
-__device__ __forceinline__ void repair_pull_arc(const Dev& D, int v, int h, int e, int w,
-                                                Tally& t) {
-  const int d = D.res[e];
-  if (d > 0 && D.height[w] > h + 1) {
-    D.rres[e] += d;
-    D.res[e] = 0;
-    const int p = D.rev[e];
-    D.rres[p] -= d;
-    D.res[p] += d;
-    add64(&D.excess[v], -static_cast<long long>(d));
-    add64(&D.excess[w], d);
-    t.repair += 1;
-    t.atom += 2;
-  }
-}
-
+// Label repair of arc e = v->w (v listed, h = h(v)), as in
+// staticMaxFlow_kernel_17 (code/dyn_data.h:140-150): residual w->v exists but
+// h(w) > h(v)+1, so saturate w->v; with the pull rule, remove_pull
+// (code/dyn_pp.h:381-403): residual v->w exists but h(w) > h(v)+1, so
+// saturate v->w.  Heights are not written during this phase, so at most one
+// endpoint of an arc pair acts.
 // Repair filters: all neighbours, B neighbours (pp_remove_push), A neighbours
 // with the pull rule (remove_pull), part neighbours (remove_push_final).
 enum RepairMode { RM_ALL = 0, RM_B = 1, RM_PULL_A = 2, RM_PFINAL = 3 };
 
-__device__ __forceinline__ void repair_one(const Dev& D, int rmode, int v, int h, int e, Tally& t) {
-  const int w = D.col[e];
-  t.arc += 1;
-  if (rmode == RM_B && !D.side[w]) return;
-  if (rmode == RM_PULL_A && D.side[w]) return;
-  if (rmode == RM_PFINAL && !D.pfinal[w]) return;
-  if (rmode == RM_PULL_A) repair_pull_arc(D, v, h, e, w, t);
-  else repair_arc(D, v, h, e, w, t);
+// Repair of arc e = v->w whose residual d > 0 was read and whose head is too
+// high.
+__device__ __forceinline__ void repair_apply(const Dev& D, int rmode, int v, int e, int w, int d, Tally& t) {
+  const int p = D.rev[e];
+  if (rmode == RM_PULL_A) {
+    D.rres[e] += d;
+    D.res[e] = 0;
+    D.rres[p] -= d;
+    D.res[p] += d;
+    add64(&D.excess[v], -static_cast<long long>(d));
+    add64(&D.excess[w], d);
+  } else {
+    D.rres[e] = 0;
+    D.res[e] += d;
+    D.rres[p] += d;
+    D.res[p] -= d;
+    add64(&D.excess[w], -static_cast<long long>(d));
+    add64(&D.excess[v], d);
+  }
+  t.repair += 1;
+  t.atom += 2;
+}
+
+// The residual a repair of arc e = v->w looks at (0 when the mode's filter
+// excludes w).
+__device__ __forceinline__ int repair_resid(const Dev& D, int rmode, int e, int w) {
+  if (w < 0) return 0;
+  if (rmode == RM_B && !D.side[w]) return 0;
+  if (rmode == RM_PULL_A && D.side[w]) return 0;
+  if (rmode == RM_PFINAL && !D.pfinal[w]) return 0;
+  return rmode == RM_PULL_A ? D.res[e] : D.rres[e];
 }
 
 // Lane path of the repair: four arcs at a time, heads, residuals and head
 // heights loaded before any of them is tested (one lane keeps several loads in
-// flight); each arc is then repaired exactly as repair_one would.
+// flight); each arc is then tested and repaired.
 __device__ __forceinline__ void repair_lane(const Dev& D, int rmode, int v, int h, int b, int end, Tally& t) {
   for (int e0 = b; e0 < end; e0 += 4) {
     int wv[4], dv[4], hw[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) wv[j] = e0 + j < end ? D.col[e0 + j] : -1;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int w = wv[j];
-      bool ok = w >= 0;
-      if (ok && rmode == RM_B) ok = D.side[w] != 0;
-      if (ok && rmode == RM_PULL_A) ok = D.side[w] == 0;
-      if (ok && rmode == RM_PFINAL) ok = D.pfinal[w] != 0;
-      dv[j] = ok ? (rmode == RM_PULL_A ? D.res[e0 + j] : D.rres[e0 + j]) : 0;
-    }
+    for (int j = 0; j < 4; ++j) dv[j] = repair_resid(D, rmode, e0 + j, wv[j]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) hw[j] = dv[j] > 0 ? D.height[wv[j]] : 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (wv[j] >= 0) t.arc += 1;
-      if (dv[j] <= 0 || hw[j] <= h + 1) continue;
-      const int e = e0 + j, w = wv[j], d = dv[j], p = D.rev[e];
-      if (rmode == RM_PULL_A) {
-        D.rres[e] += d;
-        D.res[e] = 0;
-        D.rres[p] -= d;
-        D.res[p] += d;
-        add64(&D.excess[v], -static_cast<long long>(d));
-        add64(&D.excess[w], d);
-      } else {
-        D.rres[e] = 0;
-        D.res[e] += d;
-        D.rres[p] += d;
-        D.res[p] -= d;
-        add64(&D.excess[w], -static_cast<long long>(d));
-        add64(&D.excess[v], d);
-      }
-      t.repair += 1;
-      t.atom += 2;
+      if (dv[j] > 0 && hw[j] > h + 1) repair_apply(D, rmode, v, e0 + j, wv[j], dv[j], t);
+    }
+  }
+}
+
+// Wave path of the repair over arcs [lo, hi) of v: kRepTiles 64-arc tiles per
+// step, all their heads, residuals and head heights loaded before any is
+// tested.
+constexpr int kRepTiles = 4;
+__device__ __forceinline__ void repair_wave(const Dev& D, int rmode, int v, int h, int lo, int hi, Tally& t) {
+  for (int j0 = lo; j0 < hi; j0 += kRepTiles * kWave) {
+    int wv[kRepTiles], dv[kRepTiles], hw[kRepTiles];
+#pragma unroll
+    for (int j = 0; j < kRepTiles; ++j) {
+      const int e = j0 + j * kWave + lane_id();
+      wv[j] = e < hi ? D.col[e] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < kRepTiles; ++j) dv[j] = repair_resid(D, rmode, j0 + j * kWave + lane_id(), wv[j]);
+#pragma unroll
+    for (int j = 0; j < kRepTiles; ++j) hw[j] = dv[j] > 0 ? D.height[wv[j]] : 0;
+#pragma unroll
+    for (int j = 0; j < kRepTiles; ++j) {
+      if (wv[j] >= 0) t.arc += 1;
+      if (dv[j] > 0 && hw[j] > h + 1) repair_apply(D, rmode, v, j0 + j * kWave + lane_id(), wv[j], dv[j], t);
     }
   }
 }
@@ -1266,7 +1316,7 @@ __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int
     int a, b;
     const int v = chunk_range(D, W, c, &a, &b);
     const int h = D.height[v];
-    for (int e = a + lane_id(); e < b; e += kWave) repair_one(D, rmode, v, h, e, t);
+    repair_wave(D, rmode, v, h, a, b, t);
   }
   for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
     const int i = base + lane_id();
@@ -1275,23 +1325,13 @@ __device__ void ph_repair(const Dev& D, const Range& r, const int32_t* list, int
       v = list ? list[i] : i;
       b = D.row[v];
       deg = D.row[v + 1] - b;
-      if (deg > D.wide_t) v = -1;  // wide: through W
+      if (deg > D.lane_max) v = -1;  // through W, one wave per chunk
       else {
         h = D.height[v];
         t.vert += 1;
       }
     }
-    unsigned long long big = __ballot(v >= 0 && deg > D.lane_max);
-    while (big) {
-      const int leader = __ffsll(static_cast<long long>(big)) - 1;
-      big &= big - 1;
-      const int bv = __shfl(v, leader);
-      const int bb = __shfl(b, leader);
-      const int bd = __shfl(deg, leader);
-      const int bh = __shfl(h, leader);
-      for (int e = bb + lane_id(); e < bb + bd; e += kWave) repair_one(D, rmode, bv, bh, e, t);
-    }
-    if (v >= 0 && deg <= D.lane_max) repair_lane(D, rmode, v, h, b, b + deg, t);
+    if (v >= 0) repair_lane(D, rmode, v, h, b, b + deg, t);
   }
 }
 
@@ -1701,7 +1741,7 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
   Tally t;
   const Range r = make_range();
   if (which == 0) {
-    ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[0], wlist_static(D), t);
+    ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[0], wlist_waves(D), t);
   } else if (which == 1) {
     ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t);
   } else {
@@ -1713,7 +1753,7 @@ __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, in
                                                    unsigned long long wpk) {
   Tally t;
   const Range r = make_range();
-  if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_static(D), t);
+  if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_waves(D), t);
   else if (which == 1) ph_repair(D, r, D.wl, cnt, rmode, wlist(D, kQActive, wpk), t);
   else ph_repair(D, r, D.wl2, cnt, rmode, wlist(D, kQPull, wpk), t);
   t.flush(D.ctrs);
@@ -1893,11 +1933,11 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
         // flag slots alternate by round: the slot for the next round was last
         // read one round ago, so clearing it here cannot race with a reader.
         int32_t* fl = &D.ctrl->flag[round & 1];
-        ph_push<false>(D, r, nullptr, n, C.kc, PM_ALL, fl, wlist_static(D), t);
+        ph_push<false>(D, r, nullptr, n, C.kc, PM_ALL, fl, wlist_waves(D), t);
         G.sync();
         const int flag = ldr(fl);
         if (is_block0_thread0()) D.ctrl->flag[(round + 1) & 1] = 0;
-        ph_repair(D, r, nullptr, n, RM_ALL, wlist_static(D), t);
+        ph_repair(D, r, nullptr, n, RM_ALL, wlist_waves(D), t);
         G.sync();
         if (!flag) break;
       } else {

@@ -3,9 +3,11 @@
   python -m dyn_maxflow_amd.sched_train collect --out DIR   (needs the GPU)
   python -m dyn_maxflow_amd.sched_train train --traces DIR  (CPU; writes csrc/sched_weights.inc)
 
-collect: for a set of training workloads (R-MAT scale 15-20 with seeds 11-16
-and 128/256 grids, several batch sizes and increase/decrease mixes; none uses
-seed 2, the seed of the RMAT-20 bench graph) every batch is solved under every
+collect: for a set of training workloads (--workloads base: R-MAT scale 15-20
+with seeds 11-16 and 128/256 grids; large: R-MAT scale 20 with seeds 17 and
+18 and scale 21 with seed 19; several batch sizes and increase/decrease
+mixes; none uses seed 2, the seed of the RMAT-20 bench graph) every batch is
+solved under every
 fixed schedule in
 {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}, data-driven ones
 also with the lazy start (no relabel before the first round), each from a
@@ -42,23 +44,30 @@ VARIANTS = ["data", "topo", "pp"]
 INTERVALS = [1, 2, 3, 5, 8]
 
 
-def _workloads():
+WORKLOAD_SETS = {
+    # (kind, scale or side, seed)
+    "base": [("rmat", 15, 11), ("rmat", 16, 12), ("rmat", 17, 13), ("rmat", 18, 14), ("rmat", 19, 15),
+             ("rmat", 20, 16), ("grid", 128, 21), ("grid", 256, 22)],
+    "large": [("rmat", 20, 17), ("rmat", 20, 18), ("rmat", 21, 19)],
+}
+
+
+def _workloads(which="base"):
     import dyn_maxflow_amd as dmf
-    out = []
-    for scale, seed in ((15, 11), (16, 12), (17, 13), (18, 14), (19, 15), (20, 16)):
-        u, v, w = dmf.gen_rmat(scale, 16, seed)
-        out.append((f"rmat{scale}s{seed}", u, v, w, 1, 100))
-    for side, seed in ((128, 21), (256, 22)):
-        u, v, w = dmf.gen_grid(side, side, seed, 1, 1000)
-        out.append((f"grid{side}s{seed}", u, v, w, 1, 1000))
-    return out
+    for kind, size, seed in WORKLOAD_SETS[which]:
+        if kind == "rmat":
+            u, v, w = dmf.gen_rmat(size, 16, seed)
+            yield (f"rmat{size}s{seed}", u, v, w, 1, 100)
+        else:
+            u, v, w = dmf.gen_grid(size, size, seed, 1, 1000)
+            yield (f"grid{size}s{seed}", u, v, w, 1, 1000)
 
 
-def collect(out_dir: str, quick: bool = False, mode: int = 1) -> None:
+def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base") -> None:
     sys.path.insert(0, ROOT)
     import dyn_maxflow_amd as dmf
     os.makedirs(out_dir, exist_ok=True)
-    for name, u, v, w, lo, hi in _workloads():
+    for name, u, v, w, lo, hi in _workloads(which):
         g = dmf.build_csr(u, v, w)
         if name.startswith("grid"):
             s, t = 0, g.n - 1
@@ -107,7 +116,7 @@ def _load(trace_dir: str):
     return data
 
 
-def _examples(data, teacher="batch"):
+def _examples(data, teacher="batch", variants=(0, 1, 2)):
     """(X_relabel, y_relabel, X_variant, y_variant) from the teacher schedules.
     teacher="workload": the schedule with the lowest total time over all of a
     workload's batches teaches every batch of it (per-batch winners flip
@@ -121,6 +130,8 @@ def _examples(data, teacher="batch"):
             sm = sv.get("summary")
             if not sm or sm["entry"] != 1 or sm["scheduler"] != 0:
                 continue
+            if sm["variant"] not in variants:
+                continue
             key = (sm["variant"], sm["interval"], sm.get("lazy", 0))
             runs.setdefault(key, []).append(sv)
         if not runs:
@@ -129,9 +140,9 @@ def _examples(data, teacher="batch"):
         overall = min(runs, key=lambda k: sum(r["summary"]["device_ms"] for r in runs[k][:nb]))
         for b in range(nb):
             best = overall if teacher == "workload" else min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
-            teacher = runs[best][b]
-            batch_rows = [r for r in teacher["rows"] if r["what"] == "batch"]
-            rows = [r for r in teacher["rows"] if r["what"] == "round"]
+            run = runs[best][b]
+            batch_rows = [r for r in run["rows"] if r["what"] == "batch"]
+            rows = [r for r in run["rows"] if r["what"] == "round"]
             if batch_rows:
                 Xv.append(batch_rows[0]["feat"])
                 yv.append(best[0])
@@ -213,11 +224,11 @@ def write_weights(P, note: str) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
-def train(trace_dirs, teacher="batch") -> None:
+def train(trace_dirs, teacher="batch", variants=(0, 1, 2)) -> None:
     data = {}
     for d in trace_dirs:
         data.update({f"{d}/{k}": v for k, v in _load(d).items()})
-    Xr, yr, Xv, yv = _examples(data, teacher)
+    Xr, yr, Xv, yv = _examples(data, teacher, variants)
     print(f"[train] relabel examples {len(Xr)} (positive {yr.mean() if len(yr) else 0:.2f}), "
           f"variant examples {len(Xv)} (counts {np.bincount(yv, minlength=3).tolist()})")
     P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
@@ -234,14 +245,17 @@ def main():
     c.add_argument("--out", required=True)
     c.add_argument("--quick", action="store_true")
     c.add_argument("--mode", type=int, default=1, help="driver the traces are taken on (1 stepped, 2 graph)")
+    c.add_argument("--workloads", choices=sorted(WORKLOAD_SETS), default="base")
     t = sub.add_parser("train")
     t.add_argument("--traces", required=True, nargs="+")
     t.add_argument("--teacher", choices=["workload", "batch"], default="batch")
+    t.add_argument("--variants", default="data,topo,pp",
+                   help="variants a teacher schedule may use (comma-separated)")
     a = ap.parse_args()
     if a.cmd == "collect":
-        collect(a.out, a.quick, a.mode)
+        collect(a.out, a.quick, a.mode, a.workloads)
     else:
-        train(a.traces, a.teacher)
+        train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")))
 
 
 if __name__ == "__main__":

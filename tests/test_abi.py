@@ -68,7 +68,8 @@ def test_engine_footprint_bounds_the_arrays():
     for n, m in ((1000, 16000), (1 << 20, 31401920), (8387651, 258678280)):
         b = dmf.engine_footprint(n, m)
         assert b >= 28 * m + 31 * n  # the arc and vertex arrays alone
-        assert b <= 28 * m + 40 * n + 12 * 4 * (2 * m + 16) + 48 * (m // 64 + 1) + (1 << 21)
+        assert b <= (28 * m + 40 * n + 12 * 4 * (2 * m + 16) + 8 * (m // 64 + 1) + 48 * min(n + 1, m // 17 + 1)
+                     + (1 << 21))
     assert dmf.engine_footprint(2000, 32000) > dmf.engine_footprint(1000, 16000)
 
 

@@ -5,7 +5,7 @@
 
 collect: for a set of training workloads (--workloads base: R-MAT scale 15-20
 with seeds 11-16 and 128/256 grids; large: R-MAT scale 20 with seeds 17 and
-18 and scale 21 with seed 19; several batch sizes and increase/decrease
+18 and scale 21 with seed 19; grids: a 512 grid; several batch sizes and increase/decrease
 mixes; none uses seed 2, the seed of the RMAT-20 bench graph) every batch is
 solved under every
 fixed schedule in
@@ -49,6 +49,7 @@ WORKLOAD_SETS = {
     "base": [("rmat", 15, 11), ("rmat", 16, 12), ("rmat", 17, 13), ("rmat", 18, 14), ("rmat", 19, 15),
              ("rmat", 20, 16), ("grid", 128, 21), ("grid", 256, 22)],
     "large": [("rmat", 20, 17), ("rmat", 20, 18), ("rmat", 21, 19)],
+    "grids": [("grid", 512, 23)],
 }
 
 

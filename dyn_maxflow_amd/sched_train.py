@@ -5,7 +5,9 @@
 
 collect: for a set of training workloads (--workloads base: R-MAT scale 15-20
 with seeds 11-16 and 128/256 grids; large: R-MAT scale 20 with seeds 17 and
-18 and scale 21 with seed 19; grids: a 512 grid; several batch sizes and increase/decrease
+18 and scale 21 with seed 19; grids: a 512 grid; xlarge: R-MAT scale 22
+with seed 20; small: R-MAT scale 21 and 22 with seeds 21 and 22 under
+batches of 0.002-0.5% of the arcs; several batch sizes and increase/decrease
 mixes; none uses seed 2, the seed of the RMAT-20 bench graph) every batch is
 solved under every
 fixed schedule in
@@ -50,7 +52,12 @@ WORKLOAD_SETS = {
              ("rmat", 20, 16), ("grid", 128, 21), ("grid", 256, 22)],
     "large": [("rmat", 20, 17), ("rmat", 20, 18), ("rmat", 21, 19)],
     "grids": [("grid", 512, 23)],
+    "xlarge": [("rmat", 22, 20)],
+    "small": [("rmat", 21, 21), ("rmat", 22, 22)],
 }
+# batches as (percent of the input arcs, fraction of increases)
+BATCHES = {"small": ((0.002, 0.4), (0.01, 0.4), (0.05, 0.4), (0.2, 0.4), (0.5, 0.4), (0.02, 0.6))}
+DEFAULT_BATCHES = ((1, 0.5), (3, 0.5), (5, 0.2), (10, 0.5), (2, 0.8), (7, 0.5))
 
 
 def _workloads(which="base"):
@@ -76,8 +83,8 @@ def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base
             s, t = dmf.pick_source_sink(g.n, u, v)
         batches = []
         cur = w.copy()
-        for i, (pct, inc) in enumerate(((1, 0.5), (3, 0.5), (5, 0.2), (10, 0.5), (2, 0.8), (7, 0.5))):
-            src, dst, nw = dmf.gen_updates(u, v, cur, max(1, len(u) * pct // 100), 7000 + i,
+        for i, (pct, inc) in enumerate(BATCHES.get(which, DEFAULT_BATCHES)):
+            src, dst, nw = dmf.gen_updates(u, v, cur, max(1, int(len(u) * pct / 100)), 7000 + i,
                                            inc_frac=inc, cap_lo=lo, cap_hi=hi)
             batches.append((src, dst, nw))
         trace = os.path.join(out_dir, f"{name}.jsonl")

@@ -1,0 +1,77 @@
+"""Scheduler training on synthetic traces (CPU): which schedule teaches each
+batch, the variant restriction, and the weight file the engine compiles in."""
+import json
+import os
+
+import numpy as np
+
+from dyn_maxflow_amd import sched_train as st
+
+# two fixed schedules over three batches; per batch the fastest is A, B, A,
+# over the whole workload A (12 ms against 13 ms)
+A = (0, 1, 0)  # data, relabel every round, no lazy start
+B = (2, 2, 0)  # pp, relabel every 2 rounds
+TIMES = {A: [1.0, 10.0, 1.0], B: [5.0, 2.0, 6.0]}
+
+
+def _write_trace(path):
+    solve = 0
+    with open(path, "w") as fh:
+        for key, times in TIMES.items():
+            for b, ms in enumerate(times):
+                solve += 1
+                variant, interval, lazy = key
+                feat = [float(b), float(variant), 0, 0, 0.5, 0.5, 0, 1.0]
+                fh.write(json.dumps({"solve": solve, "what": "batch", "round": 0, "variant": variant,
+                                     "relabelled": 1, "feat": feat}) + "\n")
+                for rnd in (1, 2):
+                    fh.write(json.dumps({"solve": solve, "what": "round", "round": rnd, "variant": variant,
+                                         "relabelled": int(rnd % interval == 0), "feat": feat}) + "\n")
+                fh.write(json.dumps({"solve": solve, "what": "summary", "entry": 1, "scheduler": 0,
+                                     "variant": variant, "interval": interval, "lazy": lazy,
+                                     "device_ms": ms}) + "\n")
+
+
+def test_teacher_per_batch_and_per_workload(tmp_path):
+    _write_trace(tmp_path / "w.jsonl")
+    data = st._load(str(tmp_path))
+    _, _, _, yv = st._examples(data, "batch")
+    assert yv.tolist() == [A[0], B[0], A[0]]
+    _, _, _, yv = st._examples(data, "workload")
+    assert yv.tolist() == [A[0], A[0], A[0]]
+
+
+def test_teacher_variant_restriction(tmp_path):
+    _write_trace(tmp_path / "w.jsonl")
+    data = st._load(str(tmp_path))
+    _, _, _, yv = st._examples(data, "batch", variants=(2,))
+    assert yv.tolist() == [B[0]] * 3
+    # relabel examples: one at batch start plus one per round of each teacher run
+    Xr, yr, _, _ = st._examples(data, "batch", variants=(0,))
+    assert len(Xr) == 3 * (1 + 2) and yr.tolist() == [1, 1, 1] * 3
+
+
+def test_weights_file_layout(tmp_path, monkeypatch):
+    rng = np.random.default_rng(0)
+    P = {"w1": rng.normal(size=(st.FEAT, st.HID)), "b1": rng.normal(size=st.HID),
+         "w2": rng.normal(size=(st.HID, st.HID)), "b2": rng.normal(size=st.HID),
+         "w3": rng.normal(size=(st.HID, st.OUT)), "b3": rng.normal(size=st.OUT)}
+    out = tmp_path / "w.inc"
+    monkeypatch.setattr(st, "WEIGHTS", str(out))
+    st.write_weights(P, "test")
+    txt = out.read_text()
+    body = txt[txt.index("{") + 1:txt.rindex("}")]
+    vals = [float(x.strip().rstrip("f")) for x in body.split(",") if x.strip()]
+    n = st.FEAT * st.HID + st.HID + st.HID * st.HID + st.HID + st.HID * st.OUT + st.OUT
+    assert len(vals) == n == 1476
+    flat = np.concatenate([P[k].ravel() for k in ("w1", "b1", "w2", "b2", "w3", "b3")])
+    assert np.allclose(vals, flat.astype(np.float32))
+
+
+def test_shipped_weights_parse():
+    path = os.path.join(os.path.dirname(st.__file__), "csrc", "sched_weights.inc")
+    txt = open(path).read()
+    assert "#define DMF_SCHED_TRAINED 1" in txt
+    body = txt[txt.index("{") + 1:txt.rindex("}")]
+    vals = np.array([float(x.strip().rstrip("f")) for x in body.split(",") if x.strip()])
+    assert len(vals) == 1476 and np.all(np.isfinite(vals))

@@ -1,32 +1,35 @@
 """Trains the learned scheduler from traces the engine itself generates.
 
-  python -m dyn_maxflow_amd.sched_train collect --out DIR   (needs the GPU)
-  python -m dyn_maxflow_amd.sched_train train --traces DIR  (CPU; writes csrc/sched_weights.inc)
+  python -m dyn_maxflow_amd.sched_train collect --out DIR [--workloads SET]   (needs the GPU)
+  python -m dyn_maxflow_amd.sched_train train --traces DIR [DIR ...] [--teacher batch|workload]
+                                              [--variants data,topo,pp]   (CPU; writes csrc/sched_weights.inc)
 
-collect: for a set of training workloads (--workloads base: R-MAT scale 15-20
-with seeds 11-16 and 128/256 grids; large: R-MAT scale 20 with seeds 17 and
-18 and scale 21 with seed 19; grids: a 512 grid; xlarge: R-MAT scale 22
-with seed 20; small: R-MAT scale 21 and 22 with seeds 21 and 22 under
-batches of 0.002-0.5% of the arcs; several batch sizes and increase/decrease
-mixes; none uses seed 2, the seed of the RMAT-20 bench graph) every batch is
-solved under every
-fixed schedule in
-{data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds}, data-driven ones
-also with the lazy start (no relabel before the first round), each from a
-push-pull static state so that every variant may take the batches.  The engine writes one
-trace line per scheduler evaluation (the 8 device-computed features, the round,
+collect: for one set of training workloads, every batch is solved under every
+fixed schedule in {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds},
+data-driven ones also with the lazy start (no relabel before the first
+round), each from a push-pull static state so that every variant may take
+the batches.  Sets (WORKLOAD_SETS): base = R-MAT scale 15-20 with seeds 11-16
+and 128/256 grids; large = R-MAT scale 20 with seeds 17 and 18 and scale 21
+with seed 19; grids = a 512 grid; xlarge = R-MAT scale 22 with seed 20;
+small = R-MAT scale 21 and 22 with seeds 21 and 22.  Batches are 1-10% of the
+input arcs with several increase/decrease mixes, except for small (0.002-0.5%,
+40-60% increases).  None uses seed 2 (the RMAT-20 bench graph), seed 3 (the
+evaluation grids) or the scale-23 serving graph.  The engine writes one trace
+line per scheduler evaluation (the 8 device-computed features, the round,
 whether that round started with a global relabel) and one summary line per
 solve (device ms, variant, interval).
 
-train: the fastest schedule of each batch is its teacher (--teacher workload:
-the schedule with the lowest total time over all of a workload's batches).
-The variant head learns that schedule's variant from the batch-start features; the
-relabel head learns whether the teacher relabelled before its first round
-(from the batch-start features; data-driven schedules may start on the labels
-of the previous solve) and, from the features at the end of each round of the
-teacher's run, whether it relabelled before the next round.  A
-numpy MLP 8->32->32->4 (ReLU) is fitted with Adam; the weights are written in
-the row-major SchedWeights order of csrc/sched.hpp.
+train: the fastest schedule of each batch, among the variants --variants
+allows, is its teacher (--teacher workload: the schedule with the lowest total
+time over all of a workload's batches).  The variant head learns that
+schedule's variant from the batch-start features; the relabel head learns
+whether the teacher relabelled before its first round (from the batch-start
+features; data-driven schedules may start on the labels of the previous
+solve) and, from the features at the end of each round of the teacher's run,
+whether it relabelled before the next round.  A numpy MLP 8->32->32->4 (ReLU)
+is fitted with Adam; the weights are written in the row-major SchedWeights
+order of csrc/sched.hpp.  The shipped weights come from all five sets on the
+stepped driver (collect --mode 1), all variants, per-batch teachers.
 """
 from __future__ import annotations
 

@@ -344,3 +344,56 @@ def test_lazy_start_keeps_results(gpu, mode, interval):
             applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 800 + b))
             f = eng.apply_batch(*applied[-1], p)
             check_engine(eng, g, caps_after(g, applied), s, t, f, f"lazy batch {b}")
+
+
+def degree_boundary_graph(dmf, seed=5):
+    """s -> hubs whose adjacency degree (CSR row, reverse arcs included) is
+    exactly 16/17, 64/65, 256/257 and 512/513: the default lane threshold, the
+    BFS wide threshold, one 256-arc chunk vs two, two chunks vs three.  Hubs
+    feed a pool of leaves that drain into t and into each other, so every hub
+    is active in the push phases and on the BFS frontier."""
+    rng = np.random.default_rng(seed)
+    degs = [16, 17, 64, 65, 256, 257, 512, 513]
+    s, t = 0, 1
+    hubs = list(range(2, 2 + len(degs)))
+    nleaf = 700
+    leaf0 = 2 + len(degs)
+    u, v, w = [], [], []
+    for h, d in zip(hubs, degs):
+        u.append(s); v.append(h); w.append(int(rng.integers(2000, 6000)))
+        for x in rng.choice(nleaf, size=d - 1, replace=False):  # d-1 out-arcs + the arc from s
+            u.append(h); v.append(leaf0 + int(x)); w.append(int(rng.integers(1, 40)))
+    for x in range(nleaf):
+        u.append(leaf0 + x); v.append(t); w.append(int(rng.integers(1, 60)))
+    pairs = set()
+    while len(pairs) < 1500:
+        a, b = (int(z) for z in rng.integers(0, nleaf, size=2))
+        if a != b and (b, a) not in pairs:
+            pairs.add((a, b))
+    for a, b in sorted(pairs):
+        u.append(leaf0 + a); v.append(leaf0 + b); w.append(int(rng.integers(1, 30)))
+    u, v, w = (np.array(x, np.int32) for x in (u, v, w))
+    g = dmf.build_csr(u, v, w)
+    deg = np.diff(g.row)
+    assert [int(deg[h]) for h in hubs] == degs
+    return g, s, t, (u, v, w)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_degree_boundaries(gpu, variant, mode):
+    dmf = gpu
+    g, s, t, (u, v, w) = degree_boundary_graph(dmf)
+    p = dmf.default_params(g, variant=variant, mode=mode)
+    say(f"[degrees] variant={variant} mode={mode} static")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b in range(3):
+            src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 10, 500 + b)
+            applied.append((src, dst, nw))
+            say(f"[degrees] variant={variant} mode={mode} batch {b} k={len(src)}")
+            f = eng.apply_batch(src, dst, nw, p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")

@@ -1,6 +1,7 @@
 """Trains the learned scheduler from traces the engine itself generates.
 
-  python -m dyn_maxflow_amd.sched_train collect --out DIR [--workloads SET]   (needs the GPU)
+  python -m dyn_maxflow_amd.sched_train collect --out DIR [--workloads SET | --graph G.bin --source S --sink T]
+                                                                             (needs the GPU)
   python -m dyn_maxflow_amd.sched_train train --traces DIR [DIR ...] [--teacher batch|workload]
                                               [--variants data,topo,pp]   (CPU; writes csrc/sched_weights.inc)
 
@@ -74,19 +75,38 @@ def _workloads(which="base"):
             yield (f"grid{size}s{seed}", u, v, w, 1, 1000)
 
 
-def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base") -> None:
+def graph_workload(path: str):
+    """A user's graph (.bin, the reference's graph format) as a training
+    workload: its arcs with positive capacity are the edges the update batches
+    draw from, with new capacities in [1, max capacity]."""
+    import dyn_maxflow_amd as dmf
+    g = dmf.read_graph_bin(path)
+    keep = g.cap > 0
+    u = g.tails()[keep].astype(np.int32)
+    v = g.col[keep].astype(np.int32)
+    w = g.cap[keep].astype(np.int32)
+    name = os.path.splitext(os.path.basename(path))[0]
+    return name, u, v, w, 1, int(max(1, w.max(initial=1)))
+
+
+def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base", graph: str = None,
+            source: int = None, sink: int = None) -> None:
+    """graph: collect on this .bin graph (source/sink given) instead of a set."""
     sys.path.insert(0, ROOT)
     import dyn_maxflow_amd as dmf
     os.makedirs(out_dir, exist_ok=True)
-    for name, u, v, w, lo, hi in _workloads(which):
+    workloads = [graph_workload(graph)] if graph else _workloads(which)
+    for name, u, v, w, lo, hi in workloads:
         g = dmf.build_csr(u, v, w)
-        if name.startswith("grid"):
+        if graph:
+            s, t = source, sink
+        elif name.startswith("grid"):
             s, t = 0, g.n - 1
         else:
             s, t = dmf.pick_source_sink(g.n, u, v)
         batches = []
         cur = w.copy()
-        for i, (pct, inc) in enumerate(BATCHES.get(which, DEFAULT_BATCHES)):
+        for i, (pct, inc) in enumerate(BATCHES.get("custom" if graph else which, DEFAULT_BATCHES)):
             src, dst, nw = dmf.gen_updates(u, v, cur, max(1, int(len(u) * pct / 100)), 7000 + i,
                                            inc_frac=inc, cap_lo=lo, cap_hi=hi)
             batches.append((src, dst, nw))
@@ -257,6 +277,9 @@ def main():
     c.add_argument("--quick", action="store_true")
     c.add_argument("--mode", type=int, default=1, help="driver the traces are taken on (1 stepped, 2 graph)")
     c.add_argument("--workloads", choices=sorted(WORKLOAD_SETS), default="base")
+    c.add_argument("--graph", default=None, help="collect on this .bin graph instead of a workload set")
+    c.add_argument("--source", type=int, default=None)
+    c.add_argument("--sink", type=int, default=None)
     t = sub.add_parser("train")
     t.add_argument("--traces", required=True, nargs="+")
     t.add_argument("--teacher", choices=["workload", "batch"], default="batch")
@@ -264,7 +287,9 @@ def main():
                    help="variants a teacher schedule may use (comma-separated)")
     a = ap.parse_args()
     if a.cmd == "collect":
-        collect(a.out, a.quick, a.mode, a.workloads)
+        if a.graph and (a.source is None or a.sink is None):
+            ap.error("--graph needs --source and --sink")
+        collect(a.out, a.quick, a.mode, a.workloads, a.graph, a.source, a.sink)
     else:
         train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")))
 

@@ -75,3 +75,16 @@ def test_shipped_weights_parse():
     body = txt[txt.index("{") + 1:txt.rindex("}")]
     vals = np.array([float(x.strip().rstrip("f")) for x in body.split(",") if x.strip()])
     assert len(vals) == 1476 and np.all(np.isfinite(vals))
+
+
+def test_graph_workload_from_bin(tmp_path, golden_dir):
+    """A user's .bin graph becomes a training workload: its positive-capacity
+    arcs, unchanged, as the edges batches draw from."""
+    import dyn_maxflow_amd as dmf
+    path = os.path.join(golden_dir, "johnson.bin")
+    name, u, v, w, lo, hi = st.graph_workload(path)
+    g = dmf.read_graph_bin(path)
+    assert name == "johnson" and lo == 1 and hi == int(g.cap.max())
+    assert len(u) == int((g.cap > 0).sum()) and np.all(w > 0)
+    g2 = dmf.build_csr(u, v, w)
+    assert g2.n == g.n and int(g2.cap.sum()) == int(g.cap.sum())

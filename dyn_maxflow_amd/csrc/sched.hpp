@@ -7,6 +7,8 @@
 //
 // Features (all scaled to roughly [0,1]):
 //   f0 active vertices after the round      log2(1+a) / log2(1+n)
+//      (at batch start, where f0-f3 have no round to describe: the graph's
+//      size, log2(1+n) / 32)
 //   f1 pushes in the round                  log2(1+p) / log2(1+m)
 //   f2 local relabels in the round          log2(1+r) / log2(1+n)
 //   f3 change of excess(t) in the round     sign(d) log2(1+|d|) / 32
@@ -65,7 +67,9 @@ __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, S
   float f[kFeat];
   const float ln = log2p1(D.n), lm = log2p1(D.m);
   if (phase == 0) {
-    f[0] = f[1] = f[2] = f[3] = 0.f;
+    // batch start: no round counters yet; slot 0 carries the graph's size
+    f[0] = log2p1(D.n) / 32.f;
+    f[1] = f[2] = f[3] = 0.f;
     f[6] = 0.f;
     f[7] = log2p1(last_levels) / 8.f;
   } else {

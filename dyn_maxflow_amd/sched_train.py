@@ -3,7 +3,10 @@
   python -m dyn_maxflow_amd.sched_train collect --out DIR [--workloads SET | --graph G.bin --source S --sink T]
                                                                              (needs the GPU)
   python -m dyn_maxflow_amd.sched_train train --traces DIR [DIR ...] [--teacher batch|workload]
-                                              [--variants data,topo,pp]   (CPU; writes csrc/sched_weights.inc)
+                                              [--variants data,topo,pp] [--variant-teacher same|workload]
+                                              (CPU; writes csrc/sched_weights.inc)
+  python -m dyn_maxflow_amd.sched_train retrofit --traces DIR --out DIR
+      (traces taken before feature 0 carried the graph's size at batch start)
 
 collect: for one set of training workloads, every batch is solved under every
 fixed schedule in {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds},
@@ -30,7 +33,8 @@ solve) and, from the features at the end of each round of the teacher's run,
 whether it relabelled before the next round.  A numpy MLP 8->32->32->4 (ReLU)
 is fitted with Adam; the weights are written in the row-major SchedWeights
 order of csrc/sched.hpp.  The shipped weights come from all five sets on the
-stepped driver (collect --mode 1), all variants, per-batch teachers.
+stepped driver (collect --mode 1), retrofitted, all variants, per-batch
+teachers.
 """
 from __future__ import annotations
 
@@ -130,6 +134,37 @@ def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base
                         print(f"[collect] {name} {variant} interval={interval} lazy={lazy}", flush=True)
 
 
+def _graph_size(name: str) -> int:
+    """Vertex count of a training workload, regenerated from its name."""
+    import re
+    import dyn_maxflow_amd as dmf
+    m = re.match(r"rmat(\d+)s(\d+)$", name)
+    if m:
+        u, v, w = dmf.gen_rmat(int(m.group(1)), 16, int(m.group(2)))
+        return dmf.build_csr(u, v, w).n
+    m = re.match(r"grid(\d+)s(\d+)$", name)
+    if m:
+        return int(m.group(1)) ** 2
+    raise ValueError(f"unknown workload {name}")
+
+
+def retrofit(src_dir: str, out_dir: str) -> None:
+    """Rewrites traces taken before feature 0 described the graph at batch
+    start (it was 0 there): each batch row gets log2(1+n)/32, the value the
+    engine now computes, with n regenerated from the workload's name."""
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(glob.glob(os.path.join(src_dir, "*.jsonl"))):
+        name = os.path.basename(f)[:-len(".jsonl")]
+        val = round(float(np.float32(np.log2(1.0 + _graph_size(name))) / np.float32(32.0)), 6)
+        with open(f) as fi, open(os.path.join(out_dir, os.path.basename(f)), "w") as fo:
+            for line in fi:
+                rec = json.loads(line)
+                if rec.get("what") == "batch":
+                    rec["feat"][0] = val
+                fo.write(json.dumps(rec, separators=(",", ":")) + "\n")
+        print(f"[retrofit] {name}: feature 0 at batch start = {val}")
+
+
 def _load(trace_dir: str):
     """Returns per-file lists of solves: dict(summary, rows)."""
     files = sorted(glob.glob(os.path.join(trace_dir, "*.jsonl")))
@@ -147,12 +182,15 @@ def _load(trace_dir: str):
     return data
 
 
-def _examples(data, teacher="batch", variants=(0, 1, 2)):
+def _examples(data, teacher="batch", variants=(0, 1, 2), variant_teacher="same"):
     """(X_relabel, y_relabel, X_variant, y_variant) from the teacher schedules.
     teacher="workload": the schedule with the lowest total time over all of a
     workload's batches teaches every batch of it (per-batch winners flip
     between near-equal schedules on timing noise); "batch": the fastest
-    schedule of each batch."""
+    schedule of each batch.  variant_teacher="workload": every batch of a
+    workload is taught the variant of that workload's best schedule, and its
+    relabel decisions come from the fastest run of that variant on the batch
+    (per-batch variant winners are noisier than the relabel timing)."""
     Xr, yr, Xv, yv = [], [], [], []
     for fname, solves in data.items():
         # batch solves grouped by schedule, in order; static solves (entry 0) skipped
@@ -170,7 +208,11 @@ def _examples(data, teacher="batch", variants=(0, 1, 2)):
         nb = min(len(r) for r in runs.values())
         overall = min(runs, key=lambda k: sum(r["summary"]["device_ms"] for r in runs[k][:nb]))
         for b in range(nb):
-            best = overall if teacher == "workload" else min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
+            if variant_teacher == "workload":
+                cand = [k for k in runs if k[0] == overall[0]]
+                best = min(cand, key=lambda k: runs[k][b]["summary"]["device_ms"])
+            else:
+                best = overall if teacher == "workload" else min(runs, key=lambda k: runs[k][b]["summary"]["device_ms"])
             run = runs[best][b]
             batch_rows = [r for r in run["rows"] if r["what"] == "batch"]
             rows = [r for r in run["rows"] if r["what"] == "round"]
@@ -255,11 +297,11 @@ def write_weights(P, note: str) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
-def train(trace_dirs, teacher="batch", variants=(0, 1, 2)) -> None:
+def train(trace_dirs, teacher="batch", variants=(0, 1, 2), variant_teacher="same") -> None:
     data = {}
     for d in trace_dirs:
         data.update({f"{d}/{k}": v for k, v in _load(d).items()})
-    Xr, yr, Xv, yv = _examples(data, teacher, variants)
+    Xr, yr, Xv, yv = _examples(data, teacher, variants, variant_teacher)
     print(f"[train] relabel examples {len(Xr)} (positive {yr.mean() if len(yr) else 0:.2f}), "
           f"variant examples {len(Xv)} (counts {np.bincount(yv, minlength=3).tolist()})")
     P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
@@ -280,18 +322,25 @@ def main():
     c.add_argument("--graph", default=None, help="collect on this .bin graph instead of a workload set")
     c.add_argument("--source", type=int, default=None)
     c.add_argument("--sink", type=int, default=None)
+    rf = sub.add_parser("retrofit")
+    rf.add_argument("--traces", required=True)
+    rf.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--traces", required=True, nargs="+")
     t.add_argument("--teacher", choices=["workload", "batch"], default="batch")
+    t.add_argument("--variant-teacher", choices=["same", "workload"], default="same",
+                   help="workload: each workload's best variant teaches all its batches")
     t.add_argument("--variants", default="data,topo,pp",
                    help="variants a teacher schedule may use (comma-separated)")
     a = ap.parse_args()
-    if a.cmd == "collect":
+    if a.cmd == "retrofit":
+        retrofit(a.traces, a.out)
+    elif a.cmd == "collect":
         if a.graph and (a.source is None or a.sink is None):
             ap.error("--graph needs --source and --sink")
         collect(a.out, a.quick, a.mode, a.workloads, a.graph, a.source, a.sink)
-    else:
-        train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")))
+    elif a.cmd == "train":
+        train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")), a.variant_teacher)
 
 
 if __name__ == "__main__":

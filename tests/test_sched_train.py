@@ -88,3 +88,29 @@ def test_graph_workload_from_bin(tmp_path, golden_dir):
     assert len(u) == int((g.cap > 0).sum()) and np.all(w > 0)
     g2 = dmf.build_csr(u, v, w)
     assert g2.n == g.n and int(g2.cap.sum()) == int(g.cap.sum())
+
+
+def test_variant_teacher_per_workload(tmp_path):
+    """Variant from the workload's best schedule (A: data) for every batch;
+    relabel examples from the fastest run of that variant on each batch."""
+    _write_trace(tmp_path / "w.jsonl")
+    data = st._load(str(tmp_path))
+    Xr, yr, Xv, yv = st._examples(data, "batch", variant_teacher="workload")
+    assert yv.tolist() == [A[0]] * 3
+    assert len(Xr) == 3 * 3 and yr.tolist() == [1, 1, 1] * 3  # A relabels every round
+
+
+def test_retrofit_sets_graph_size_at_batch_start(tmp_path):
+    src = tmp_path / "src"
+    src.mkdir()
+    _write_trace(src / "grid64s9.jsonl")  # a 64x64 grid: n = 4096
+    st.retrofit(str(src), str(tmp_path / "out"))
+    rows = [json.loads(l) for l in open(tmp_path / "out" / "grid64s9.jsonl")]
+    orig = [json.loads(l) for l in open(src / "grid64s9.jsonl")]
+    want = round(float(np.float32(np.log2(4097.0)) / np.float32(32.0)), 6)
+    assert len(rows) == len(orig)
+    for r, o in zip(rows, orig):
+        if r["what"] == "batch":
+            assert r["feat"][0] == want and r["feat"][1:] == o["feat"][1:]
+        else:
+            assert r == o  # round and summary rows are unchanged

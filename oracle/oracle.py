@@ -1,8 +1,9 @@
 """ctypes wrapper of oracle/liboracle.so — TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
-this module, and only as the checker or as the timed CPU baseline.  The engine
-in dyn_maxflow_amd/ never imports it.
+this module, and only to compute the expected results the engine is compared
+with or as the timed CPU baseline.  The engine in dyn_maxflow_amd/ never
+imports it.
 
 See oracle.c for what each function restates (reference file:line) and how it
 is pinned.

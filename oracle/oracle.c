@@ -1,6 +1,6 @@
 /*
  * oracle.c — CPU restatement of the reference's dynamic push-relabel, plus an
- * int64 Dinic max-flow, used ONLY as a checker.
+ * int64 Dinic max-flow, used ONLY to compute expected results in tests.
  *
  * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load this library, and only to check or to time a CPU

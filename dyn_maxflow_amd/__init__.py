@@ -172,6 +172,7 @@ def _lib():
         "dmf_strerror": (C.c_char_p, [C.c_int]),
         "dmf_free": (None, [C.c_void_p]),
         "dmf_csr_free": (None, [C.POINTER(_Csr)]),
+        "dmf_csr_check": (C.c_int, [C.POINTER(_Csr)]),
         "dmf_read_graph_bin": (C.c_int, [C.c_char_p, C.POINTER(_Csr)]),
         "dmf_write_graph_bin": (C.c_int, [C.c_char_p, C.POINTER(_Csr)]),
         "dmf_build_csr": (C.c_int, [C.c_int64, _I32P, _I32P, _I32P, C.POINTER(_Csr)]),
@@ -288,6 +289,9 @@ class Graph:
         self.col = _i32(self.col)
         self.cap = _i32(self.cap)
         self.rev = _i32(self.rev)
+        if (self.n < 0 or self.m < 0 or self.row.shape != (self.n + 1,)
+                or any(a.shape != (self.m,) for a in (self.col, self.cap, self.rev))):
+            raise DmfError(EINVAL, f"CSR arrays do not match n={self.n}, m={self.m}")
         return _Csr(self.n, self.m, _ptr(self.row), _ptr(self.col), _ptr(self.cap), _ptr(self.rev))
 
     @classmethod
@@ -302,6 +306,13 @@ class Graph:
 
     def tails(self) -> np.ndarray:
         return np.repeat(np.arange(self.n, dtype=np.int32), np.diff(self.row))
+
+
+def csr_check(g: Graph) -> None:
+    """Structural check of a CSR (dmf_csr_check); raises DmfError(EFORMAT) on a
+    malformed graph.  Engine creation runs the same check itself."""
+    c = g._csr()
+    _check(_lib().dmf_csr_check(C.byref(c)), "csr_check")
 
 
 def read_graph_bin(path: str) -> Graph:

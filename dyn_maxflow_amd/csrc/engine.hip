@@ -923,6 +923,8 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
   if (!g || !out || g->n < 2 || g->m < 0 || s < 0 || s >= g->n || t < 0 || t >= g->n || s == t)
     return DMF_EINVAL;
+  *out = nullptr;
+  if (const int rc = dmf_csr_check(g)) return rc;
   const std::vector<int32_t> srcs{s}, snks{t};
   return create_engine(g, srcs, snks, nullptr, 1, out);
 }
@@ -937,6 +939,7 @@ int dmf_batch_create(int32_t ninst, const dmf_csr* graphs, const int32_t* s, con
     const dmf_csr& gi = graphs[i];
     if (gi.n < 2 || gi.m < 0 || s[i] < 0 || s[i] >= gi.n || t[i] < 0 || t[i] >= gi.n || s[i] == t[i])
       return DMF_EINVAL;
+    if (const int rc = dmf_csr_check(&gi)) return rc;
     voff[i + 1] = voff[i] + gi.n;
     aoff[i + 1] = aoff[i] + gi.m;
   }

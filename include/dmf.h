@@ -26,7 +26,7 @@ extern "C" {
 #define DMF_ENOMEM (-4)   /* host or device allocation failed                   */
 #define DMF_EDEVICE (-5)  /* HIP runtime error                                  */
 #define DMF_ENOCONV (-6)  /* round cap reached before convergence               */
-#define DMF_EFORMAT (-7)  /* file contents inconsistent (negative sizes, ...)   */
+#define DMF_EFORMAT (-7)  /* file or CSR inconsistent (negative sizes, ...)     */
 #define DMF_EADMIT (-8)   /* serving: request exceeds the admission budget      */
 #define DMF_EAGAIN (-9)   /* serving: result not ready within the timeout       */
 
@@ -56,6 +56,15 @@ typedef struct dmf_csr {
 void dmf_csr_free(dmf_csr* g);
 /* Release any array returned by a dmf_* producer (update arrays, edge lists). */
 void dmf_free(void* p);
+
+/* Structural check of a CSR; the reference has none (its readers trust the
+ * file).  Offsets start at 0, never decrease and end at m; heads are in
+ * [0, n); capacities are >= 0; rev[e] is in [0, m), rev[rev[e]] == e, and the
+ * head of rev[e] is the tail of e.  DMF_OK, DMF_EINVAL (null arrays, negative
+ * sizes) or DMF_EFORMAT.  dmf_read_graph_bin and every entry point that builds
+ * an engine from a caller's CSR run it first, so a malformed graph is an error
+ * status and never an out-of-range index on the device.  One host pass, O(n + m). */
+int dmf_csr_check(const dmf_csr* g);
 
 /* Replaces graph::loadGraph (code/graph.hpp:202-223).  Layout, all int32 LE:
  * [n][m][row n+1][col m][cap m][rev m].  Unlike the reference, short reads and

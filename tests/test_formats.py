@@ -155,6 +155,78 @@ def test_malformed_files_are_reported(tmp_path, golden_dir):
     assert e.value.status == dmf.EIO
 
 
+def test_csr_check_rejects_each_malformation(golden_dir):
+    """Every golden graph passes; each single corruption that would send a
+    kernel out of range, or pair an arc with the wrong reverse, is EFORMAT."""
+    for name in ("johnson.bin", "rmat8.bin"):  # graphs written by the reference's create_bin
+        dmf.csr_check(dmf.read_graph_bin(os.path.join(golden_dir, name)))
+    g = dmf.build_csr([0, 1, 2, 0], [1, 2, 3, 2], [5, 4, 3, 2])
+    dmf.csr_check(g)
+    tails = g.tails()
+    # an arc whose reverse is itself a valid involution but heads to the wrong tail
+    e = int(np.flatnonzero(tails == 0)[0])
+    other = int(np.flatnonzero((tails != 0) & (g.col != 0))[0])
+    swap = g.rev.copy()
+    a, b = e, other
+    ra, rb = swap[a], swap[b]
+    swap[a], swap[b], swap[ra], swap[rb] = rb, ra, b, a  # still rev[rev[x]] == x
+    assert np.array_equal(swap[swap], np.arange(g.m))
+    bad = {
+        "row start": ("row", 0, 1),
+        "row end": ("row", g.n, g.m - 1),
+        "row order": ("row", 2, int(g.row[3]) + 1),
+        "head range": ("col", 1, g.n),
+        "negative head": ("col", 1, -1),
+        "negative cap": ("cap", 0, -1),
+        "rev range": ("rev", 0, g.m),
+        "rev involution": ("rev", 0, (int(g.rev[0]) + 1) % g.m),
+    }
+    for what, (field, i, val) in bad.items():
+        h = dmf.Graph(g.n, g.m, g.row.copy(), g.col.copy(), g.cap.copy(), g.rev.copy())
+        getattr(h, field)[i] = val
+        with pytest.raises(dmf.DmfError) as err:
+            dmf.csr_check(h)
+        assert err.value.status == dmf.EFORMAT, what
+    h = dmf.Graph(g.n, g.m, g.row.copy(), g.col.copy(), g.cap.copy(), swap)
+    with pytest.raises(dmf.DmfError) as err:
+        dmf.csr_check(h)
+    assert err.value.status == dmf.EFORMAT
+    with pytest.raises(dmf.DmfError) as err:
+        dmf.csr_check(dmf.Graph(g.n, g.m, g.row[:-1], g.col, g.cap, g.rev))
+    assert err.value.status == dmf.EINVAL
+
+
+def test_csr_check_threaded_ranges():
+    """Above 2^22 arcs the arc pass is split over threads: a corruption in the
+    first, a middle and the last arc is found whichever range holds it."""
+    u, v, w = dmf.gen_rmat(18, 16, 3)
+    g = dmf.build_csr(u, v, w)
+    assert g.m >= 1 << 22
+    dmf.csr_check(g)
+    for e in (0, g.m // 2 + 1, g.m - 1):
+        rev = g.rev.copy()
+        rev[e] = (rev[e] + 1) % g.m
+        with pytest.raises(dmf.DmfError) as err:
+            dmf.csr_check(dmf.Graph(g.n, g.m, g.row, g.col, g.cap, rev))
+        assert err.value.status == dmf.EFORMAT
+
+
+def test_engine_rejects_malformed_csr_before_any_device_call():
+    """dmf_engine_create, dmf_batch_create and the recalculate entry point run
+    the CSR check first, so a bad graph is EFORMAT with or without a GPU."""
+    g = dmf.build_csr([0, 1, 2, 0], [1, 2, 3, 2], [5, 4, 3, 2])
+    g.col[1] = g.n + 5
+    with pytest.raises(dmf.DmfError) as err:
+        dmf.Engine(g, 0, 3)
+    assert err.value.status == dmf.EFORMAT
+    with pytest.raises(dmf.DmfError) as err:
+        dmf.BatchEngine([g], [0], [3])
+    assert err.value.status == dmf.EFORMAT
+    with pytest.raises(dmf.DmfError) as err:
+        dmf.recalculate_max_flow(g, [0], [1], [7], 0, 3, kc=1)
+    assert err.value.status == dmf.EFORMAT
+
+
 def test_rmat_generator_properties():
     u, v, w = dmf.gen_rmat(10, 16, 1)
     u2, v2, w2 = dmf.gen_rmat(10, 16, 1)

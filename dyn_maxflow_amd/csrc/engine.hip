@@ -578,7 +578,9 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
   c.variant = p->variant;
   c.kc = p->kc > 0 ? p->kc : 1;
   c.interval = p->relabel_interval > 0 ? p->relabel_interval : 1;
-  c.max_rounds = p->max_rounds > 0 ? p->max_rounds : 4 * E->n + 1000;
+  c.max_rounds = p->max_rounds > 0
+                     ? p->max_rounds
+                     : static_cast<int>(std::min<int64_t>(4LL * E->n + 1000, INT32_MAX));
   c.entry = entry;
   c.scheduler = p->scheduler;
   c.lazy = p->lazy_start;
@@ -637,6 +639,7 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
 int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   Cfg C = make_cfg(E, p, entry);
   if (C.variant < 0 || C.variant > 2 || C.scheduler < 0 || C.scheduler > 1) return DMF_EINVAL;
+  if (p && (p->mode < 0 || p->mode > 2)) return DMF_EINVAL;
   if (entry == 1 && C.variant == DMF_VARIANT_PP && !E->pp_ready) C.variant = DMF_VARIANT_DATA;
   int rc = begin_solve(E);
   if (rc) return rc;

@@ -139,6 +139,14 @@ def test_batch_edge_cases(gpu, mode):
         rc = eng.apply_batch_status([g.n + 5], [0], [1], p)
         assert rc == dmf.EINVAL
         assert eng.flow() == f0
+        # out-of-range parameters are refused before any device work
+        for field, val in (("mode", 7), ("mode", -1), ("variant", 9), ("scheduler", 4)):
+            bad = dmf.default_params(g, mode=mode)
+            setattr(bad, field, val)
+            assert eng.apply_batch_status([int(u[0])], [int(v[0])], [1], bad) == dmf.EINVAL, field
+        assert eng.flow() == f0
+        res2, cap2, _, _ = eng.state()
+        assert np.array_equal(cap2, g.cap) and np.array_equal(res2, res)
         # duplicate records for one arc: the last wins
         a, b = int(u[1]), int(v[1])
         f = eng.apply_batch([a, a, a], [b, b, b], [100, 1, 42], p)

@@ -692,17 +692,19 @@ int ensure_update_capacity(dmf_engine* E, int32_t k) {
 
 namespace {
 
-// (Re)builds the wide-vertex structures for threshold wide_t: the static list
-// of every vertex of degree > wide_t with chunk bases, its bitmap, and the
-// chunk queues sized for it.
-int setup_wide(dmf_engine* E, int32_t wide_t) {
+// (Re)builds the wide-vertex structures for thresholds lane_max and wide_t: the
+// static list of every vertex of degree > wide_t with chunk bases, its bitmap,
+// the wave list and the chunk queues sized for it.  The new lists are built in
+// full before the old ones are released, so on failure the engine keeps its
+// previous thresholds and lists intact.
+int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   Dev& D = E->D;
   std::vector<int32_t> wide_v, wide_b, wave_v, wave_b;
   int64_t wide_chunks = 0, wave_chunks = 0;
   // Vertices above lane_max: push and repair give each its own wave (one per
   // chunk), from the active queues or, topology-driven, from the static wave
   // list; the queues have a slot for every such vertex.
-  const int32_t wave_t = std::min(D.lane_max, wide_t);
+  const int32_t wave_t = std::min(lane_max, wide_t);
   for (int32_t v = 0; v < E->n; ++v) {
     const int32_t d = E->h_row[v + 1] - E->h_row[v];
     if (d > wave_t) {
@@ -720,35 +722,47 @@ int setup_wide(dmf_engine* E, int32_t wide_t) {
   const size_t nqueue = std::max<size_t>(1, wave_v.size());
   std::vector<uint32_t> wbits((static_cast<size_t>(E->n) + 31) / 32 + 1, 0u);
   for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
-  HIP_OK(hipStreamSynchronize(E->stream));
+  HIP_OK(hipStreamSynchronize(E->stream));  // no kernel uses the old lists from here on
+  int32_t *wq_vert = nullptr, *wq_base = nullptr, *wide_vert = nullptr, *wide_base = nullptr;
+  int32_t *wave_vert = nullptr, *wave_base = nullptr;
+  uint32_t* widebits = nullptr;
+  const int64_t before = E->bytes;
+  auto put = [](void* dst, const void* src, size_t bytes) {
+    return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  int rc;
+  if (!(rc = dalloc(E, &wq_vert, nqueue * kNumQ)) && !(rc = dalloc(E, &wq_base, nqueue * kNumQ)) &&
+      !(rc = dalloc(E, &wide_vert, nwide)) && !(rc = dalloc(E, &wide_base, nwide)) &&
+      !(rc = dalloc(E, &wave_vert, nqueue)) && !(rc = dalloc(E, &wave_base, nqueue)) &&
+      !(rc = dalloc(E, &widebits, wbits.size()))) {
+    const bool copied = put(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size()) &&
+                        put(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size()) &&
+                        put(wave_vert, wave_v.data(), sizeof(int32_t) * wave_v.size()) &&
+                        put(wave_base, wave_b.data(), sizeof(int32_t) * wave_b.size()) &&
+                        put(widebits, wbits.data(), sizeof(uint32_t) * wbits.size());
+    if (!copied) rc = DMF_EDEVICE;
+  }
+  if (rc) {
+    for (void* p : {static_cast<void*>(wq_vert), static_cast<void*>(wq_base), static_cast<void*>(wide_vert),
+                    static_cast<void*>(wide_base), static_cast<void*>(wave_vert),
+                    static_cast<void*>(wave_base), static_cast<void*>(widebits)})
+      if (p) (void)hipFree(p);
+    E->bytes = before;
+    return rc;
+  }
   for (void* p : {static_cast<void*>(D.wq_vert), static_cast<void*>(D.wq_base),
                   const_cast<void*>(static_cast<const void*>(D.wide_vert)),
                   const_cast<void*>(static_cast<const void*>(D.wide_base)),
                   const_cast<void*>(static_cast<const void*>(D.wave_vert)),
                   const_cast<void*>(static_cast<const void*>(D.wave_base)),
                   const_cast<void*>(static_cast<const void*>(D.widebits))})
-    if (p) HIP_OK(hipFree(p));
+    if (p) (void)hipFree(p);
+  const int64_t new_bytes = E->bytes - before;
   E->bytes -= E->wide_bytes;
-  int32_t *wq_vert = nullptr, *wq_base = nullptr, *wide_vert = nullptr, *wide_base = nullptr;
-  int32_t *wave_vert = nullptr, *wave_base = nullptr;
-  uint32_t* widebits = nullptr;
-  const int64_t before = E->bytes;
-  int rc;
-  if ((rc = dalloc(E, &wq_vert, nqueue * kNumQ)) || (rc = dalloc(E, &wq_base, nqueue * kNumQ)) ||
-      (rc = dalloc(E, &wide_vert, nwide)) || (rc = dalloc(E, &wide_base, nwide)) ||
-      (rc = dalloc(E, &wave_vert, nqueue)) || (rc = dalloc(E, &wave_base, nqueue)) ||
-      (rc = dalloc(E, &widebits, wbits.size())))
-    return rc;
-  E->wide_bytes = E->bytes - before;
-  if (!wide_v.empty()) {
-    HIP_OK(hipMemcpy(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size(), hipMemcpyHostToDevice));
-  }
-  if (!wave_v.empty()) {
-    HIP_OK(hipMemcpy(wave_vert, wave_v.data(), sizeof(int32_t) * wave_v.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(wave_base, wave_b.data(), sizeof(int32_t) * wave_b.size(), hipMemcpyHostToDevice));
-  }
-  HIP_OK(hipMemcpy(widebits, wbits.data(), sizeof(uint32_t) * wbits.size(), hipMemcpyHostToDevice));
+  E->wide_bytes = new_bytes;
+  // BFS segments captured for the old lists launch the old chunk count's grid
+  destroy_segments(E);
+  D.lane_max = lane_max;
   D.wq_vert = wq_vert;
   D.wq_base = wq_base;
   D.wq_cap = static_cast<int32_t>(nqueue);
@@ -793,18 +807,22 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
   std::vector<uint8_t> term(n, 0);
   for (int32_t v : srcs) term[v] |= 1;
   for (int32_t v : snks) term[v] |= 2;
-  int32_t *row, *col, *rev, *cap, *res, *rres, *height, *order, *wl, *wl2, *part, *owner, *hval;
-  int32_t *d_srcs, *d_snks, *d_inst = nullptr;
-  uint8_t* d_term;
-  long long* iflow;
-  unsigned long long* ctrs;
-  long long* excess;
-  uint8_t *side, *pfinal;
-  Ctrl* ctrl;
-  unsigned long long* hkey;
-  uint16_t* lev16;
+  // Every pointer starts null and reaches E->D before the failure check, so a
+  // failed allocation part way through frees what was already allocated.
+  int32_t *row = nullptr, *col = nullptr, *rev = nullptr, *cap = nullptr, *res = nullptr,
+          *rres = nullptr, *height = nullptr, *order = nullptr, *wl = nullptr, *wl2 = nullptr,
+          *part = nullptr, *owner = nullptr, *hval = nullptr;
+  int32_t *d_srcs = nullptr, *d_snks = nullptr, *d_inst = nullptr;
+  uint8_t* d_term = nullptr;
+  long long* iflow = nullptr;
+  unsigned long long* ctrs = nullptr;
+  long long* excess = nullptr;
+  uint8_t *side = nullptr, *pfinal = nullptr;
+  Ctrl* ctrl = nullptr;
+  unsigned long long* hkey = nullptr;
+  uint16_t* lev16 = nullptr;
   int rc = DMF_OK;
-  if ((rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
+  const bool alloc_failed = (rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
       (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
       (rc = dalloc(E, &rres, m)) || (rc = dalloc(E, &excess, n)) || (rc = dalloc(E, &height, n)) ||
       (rc = dalloc(E, &order, n)) || (rc = dalloc(E, &wl, n)) || (rc = dalloc(E, &wl2, n)) ||
@@ -813,16 +831,17 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
-      (rc = dalloc(E, &lev16, n)) || (inst && (rc = dalloc(E, &d_inst, n)))) {
-    dmf_engine_destroy(E);
-    return rc;
-  }
+      (rc = dalloc(E, &lev16, n)) || (inst && (rc = dalloc(E, &d_inst, n)));
   D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
   D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
   D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
   D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
   D.lev16 = lev16;
+  if (alloc_failed) {
+    dmf_engine_destroy(E);
+    return rc;
+  }
   {
     // direction-optimizing relabel: off unless DMF_BFS_BOTTOM_UP gives the
     // divisor (on residual graphs near a maximum flow the source side is
@@ -860,7 +879,7 @@ int create_engine(const dmf_csr* g, const std::vector<int32_t>& srcs, const std:
             hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
             hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess &&
             hipMemsetAsync(hval, 0xff, sizeof(int32_t) * hcap, E->stream) == hipSuccess;
-  if (!ok || setup_wide(E, kWideT) != DMF_OK) {
+  if (!ok || setup_wide(E, kLaneMax, kWideT) != DMF_OK) {
     dmf_engine_destroy(E);
     return DMF_EDEVICE;
   }
@@ -1240,8 +1259,7 @@ int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, i
   double best_ms = 1e300;
   int best = 1;
   for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {
-    E->D.lane_max = cand[c][0];
-    int rc = setup_wide(E, cand[c][1]);
+    int rc = setup_wide(E, cand[c][0], cand[c][1]);
     if (!rc) rc = dmf_engine_reset(E);
     int64_t f = 0;
     double ms = 0;
@@ -1255,8 +1273,7 @@ int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, i
       best = c;
     }
   }
-  E->D.lane_max = cand[best][0];
-  int rc = setup_wide(E, cand[best][1]);
+  int rc = setup_wide(E, cand[best][0], cand[best][1]);
   if (!rc) rc = dmf_engine_reset(E);
   if (lane_max) *lane_max = cand[best][0];
   if (wide_t) *wide_t = cand[best][1];

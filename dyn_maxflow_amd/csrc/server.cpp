@@ -271,9 +271,14 @@ int dmf_server_create(const dmf_server_config* cfg, dmf_server** out) {
   S->arena.init(S->cfg.staging_bytes);
   S->held = S->cfg.staging_bytes;
   S->t0 = std::chrono::steady_clock::now();
-  dmf_server* raw = S.release();
-  raw->stager = std::thread(stager_loop, raw);
-  *out = raw;
+  try {
+    S->stager = std::thread(stager_loop, S.get());
+  } catch (...) {  // no thread available; nothing may escape the C-ABI
+    (void)hipStreamDestroy(S->copy_stream);
+    (void)hipFree(S->d_arena);
+    return DMF_ENOMEM;
+  }
+  *out = S.release();
   return DMF_OK;
 }
 
@@ -315,7 +320,14 @@ int dmf_server_open(dmf_server* S, const dmf_csr* g, int32_t s, int32_t t, int32
   ss->id = S->next_session++;
   Session* raw = ss.get();
   S->sessions[ss->id] = std::move(ss);
-  raw->solver = std::thread(solver_loop, S, raw);
+  try {
+    raw->solver = std::thread(solver_loop, S, raw);
+  } catch (...) {  // no thread available: undo the session
+    S->held -= raw->bytes;
+    dmf_engine_destroy(raw->eng);
+    S->sessions.erase(raw->id);
+    return DMF_ENOMEM;
+  }
   *session = raw->id;
   if (static_flow) *static_flow = f;
   return DMF_OK;

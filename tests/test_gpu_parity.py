@@ -294,6 +294,8 @@ def test_autotune_keeps_results(gpu):
         p = dmf.default_params(g)
         lm, wt = eng.autotune(p)
         assert lm > 0 and wt > 0
+        # rebuilding the wide lists per candidate releases the old ones
+        assert 0 < eng.device_bytes() <= dmf.engine_footprint(g.n, g.m)
         f = eng.static_solve(p)
         check_engine(eng, g, g.cap, s, t, f, "autotuned static")
         cur = w.copy()

@@ -131,6 +131,9 @@ struct Dev {
   const int32_t* wave_base;
   unsigned long long wave_packed;
   const uint32_t* widebits;  // bit v set when degree(v) > wide_t
+  // Engine numbering of each caller vertex id, or null when the engine keeps
+  // the caller's numbering.  Update records carry caller ids.
+  const int32_t* vmap;
   // current update batch
   const int32_t* usrc;
   const int32_t* udst;
@@ -1498,7 +1501,7 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
     if (a < 0 || a >= D.n || b < 0 || b >= D.n) {
       ++invalid;
     } else if (a != b) {  // self-loops skipped (code/update.hpp:82, code/dyn_data.h:194)
-      arc = lookup_arc(D, a, b);
+      arc = D.vmap ? lookup_arc(D, D.vmap[a], D.vmap[b]) : lookup_arc(D, a, b);
       if (arc < 0) {
         ++missing;
       } else {
@@ -1555,8 +1558,8 @@ __device__ void ph_upd_clamp(const Dev& D, const Range& r, Tally& t) {
       D.rres[p] = 0;
       D.rres[arc] -= a;
       D.res[p] -= a;
-      add64(&D.excess[D.usrc[i]], a);
-      add64(&D.excess[D.udst[i]], -static_cast<long long>(a));
+      add64(&D.excess[D.col[p]], a);  // the arc's tail
+      add64(&D.excess[D.col[arc]], -static_cast<long long>(a));
       t.atom += 2;
     }
   }

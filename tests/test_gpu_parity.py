@@ -407,3 +407,38 @@ def test_degree_boundaries(gpu, variant, mode):
             say(f"[degrees] variant={variant} mode={mode} batch {b} k={len(src)}")
             f = eng.apply_batch(src, dst, nw, p)
             check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_vertex_scatter_is_invisible(gpu, mode, monkeypatch):
+    """The engine may renumber the vertices of a graph with hubs internally.
+    Flows, S_min, capacities and the residual state read back must be in the
+    caller's numbering and equal, scattered or not, through batches of every
+    variant (including arcs into and out of s and t)."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(11, 16, 4)
+    cur = w.copy()
+    bl = [dmf.gen_updates(u, v, cur, len(u) * pc // 100, 90 + i) for i, pc in enumerate((3, 6, 2))]
+    so = g.col[g.row[s]:g.row[s + 1]]
+    bl.append((np.full(len(so), s), so, np.full(len(so), 77)))
+    seqs = {}
+    for scatter in ("0", "1"):
+        monkeypatch.setenv("DMF_VERTEX_SCATTER", scatter)
+        say(f"[scatter] mode={mode} scatter={scatter}")
+        flows, cuts = [], []
+        with dmf.Engine(g, s, t) as eng:
+            f = eng.static_solve(dmf.default_params(g, variant="pp", mode=mode))
+            check_engine(eng, g, g.cap, s, t, f, f"scatter={scatter} static")
+            flows.append(f)
+            cuts.append(eng.min_cut())
+            done = []
+            for i, var in enumerate(("data", "pp", "topo", "data")):
+                f = eng.apply_batch(*bl[i], dmf.default_params(g, variant=var, mode=mode))
+                done.append(bl[i])
+                check_engine(eng, g, caps_after(g, done), s, t, f, f"scatter={scatter} batch {i} {var}")
+                flows.append(f)
+                cuts.append(eng.min_cut())
+        seqs[scatter] = (flows, cuts)
+    assert seqs["0"][0] == seqs["1"][0]
+    for a, b in zip(seqs["0"][1], seqs["1"][1]):
+        assert np.array_equal(a, b)

@@ -786,16 +786,21 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
 }
 
 // ---------------------------------------------------------------- vertex scattering
-// On graphs with hub vertices the engine numbers the vertices in a seeded random
-// order.  On RMAT-20 (edge factor 16, 10 batches of 1-10% updates) this cut the
-// mean global-relabel level from 54-55 to 48 us at the same arc visits, and the
-// plain batch by about 7%, with equal flows; a degree-sorted numbering gave
-// 53 us.  Grids and meshes keep their numbering, whose neighbour locality a
-// random order would destroy.  DMF_VERTEX_SCATTER=0/1 overrides the choice.
-constexpr int64_t kScatterSkew = 32;  // scatter when max degree >= 32 x mean degree
+// On graphs with hub vertices whose labels fit in one XCD's L2, the engine numbers
+// the vertices in a seeded random order.  On RMAT-20 (1 M vertices, edge factor
+// 16, 10 batches of 1-10% updates) this cut the mean global-relabel level from
+// 54-55 to 48 us at the same arc visits, and the plain batch by about 7%, with
+// equal flows; a degree-sorted numbering gave 53 us.  On RMAT-23 (8.4 M
+// vertices, 16 MB of 16-bit labels) streaming batches were 20% slower scattered:
+// there the generator's id locality saves L2 misses that a random order adds.
+// Grids and meshes keep their numbering, whose neighbour locality a random
+// order would destroy.  DMF_VERTEX_SCATTER=0/1 overrides the choice.
+constexpr int64_t kScatterSkew = 32;          // max degree >= 32 x mean degree
+constexpr int32_t kScatterMaxN = 1 << 21;     // 2 B labels of 2 M vertices fill 4 MB
 
 bool want_scatter(const dmf_csr* g) {
   if (const char* e = std::getenv("DMF_VERTEX_SCATTER")) return std::atoi(e) != 0;
+  if (g->n > kScatterMaxN) return false;
   int64_t maxd = 0;
   for (int32_t v = 0; v < g->n; ++v) maxd = std::max<int64_t>(maxd, g->row[v + 1] - g->row[v]);
   const int64_t mean = std::max<int64_t>(1, static_cast<int64_t>(g->m) / std::max(1, g->n));

@@ -1355,7 +1355,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   while (hcap < 2 * static_cast<uint64_t>(M) + 16) hcap <<= 1;
   int64_t b = 4 * (N + 1);                      // row
   b += 7 * 4 * M;                               // col rev cap cap0 res rres owner
-  b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst
+  b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst or vmap
   b += 2 * N;                                   // 16-bit BFS levels
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap

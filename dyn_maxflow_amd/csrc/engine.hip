@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "../../include/dmf.h"
+#include "abi_guard.hpp"
 #include "phases.hpp"
 #include "sched.hpp"
 
@@ -1084,9 +1085,11 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
 extern "C" {
 
 int dmf_device_count(void) {
-  int c = 0;
-  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
-  return c;
+  return guarded([&]() -> int {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+  });
 }
 
 void dmf_params_default(const dmf_csr* g, dmf_params* p) {
@@ -1101,251 +1104,277 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
 }
 
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
-  if (!g || !out || g->n < 2 || g->m < 0 || s < 0 || s >= g->n || t < 0 || t >= g->n || s == t)
-    return DMF_EINVAL;
-  *out = nullptr;
-  if (const int rc = dmf_csr_check(g)) return rc;
-  const std::vector<int32_t> srcs{s}, snks{t};
-  return create_engine(g, srcs, snks, nullptr, 1, out);
+  return guarded([&]() -> int {
+    if (!g || !out || g->n < 2 || g->m < 0 || s < 0 || s >= g->n || t < 0 || t >= g->n || s == t)
+      return DMF_EINVAL;
+    *out = nullptr;
+    if (const int rc = dmf_csr_check(g)) return rc;
+    const std::vector<int32_t> srcs{s}, snks{t};
+    return create_engine(g, srcs, snks, nullptr, 1, out);
+  });
 }
 
 int dmf_batch_create(int32_t ninst, const dmf_csr* graphs, const int32_t* s, const int32_t* t,
                      dmf_engine** out) {
-  if (!out || ninst < 1 || !graphs || !s || !t) return DMF_EINVAL;
-  *out = nullptr;
-  // disjoint union: instance i's vertices are shifted by voff[i], arcs by aoff[i]
-  std::vector<int64_t> voff(ninst + 1, 0), aoff(ninst + 1, 0);
-  for (int32_t i = 0; i < ninst; ++i) {
-    const dmf_csr& gi = graphs[i];
-    if (gi.n < 2 || gi.m < 0 || s[i] < 0 || s[i] >= gi.n || t[i] < 0 || t[i] >= gi.n || s[i] == t[i])
-      return DMF_EINVAL;
-    if (const int rc = dmf_csr_check(&gi)) return rc;
-    voff[i + 1] = voff[i] + gi.n;
-    aoff[i + 1] = aoff[i] + gi.m;
-  }
-  if (voff[ninst] > INT32_MAX - 1 || aoff[ninst] > INT32_MAX) return DMF_EINVAL;
-  const int32_t n = static_cast<int32_t>(voff[ninst]), m = static_cast<int32_t>(aoff[ninst]);
-  std::vector<int32_t> row(n + 1), col(m), cap(m), rev(m), inst(n), srcs(ninst), snks(ninst);
-  for (int32_t i = 0; i < ninst; ++i) {
-    const dmf_csr& gi = graphs[i];
-    const int32_t vo = static_cast<int32_t>(voff[i]), ao = static_cast<int32_t>(aoff[i]);
-    for (int32_t v = 0; v < gi.n; ++v) {
-      row[vo + v] = ao + gi.row[v];
-      inst[vo + v] = i;
+  return guarded([&]() -> int {
+    if (!out || ninst < 1 || !graphs || !s || !t) return DMF_EINVAL;
+    *out = nullptr;
+    // disjoint union: instance i's vertices are shifted by voff[i], arcs by aoff[i]
+    std::vector<int64_t> voff(ninst + 1, 0), aoff(ninst + 1, 0);
+    for (int32_t i = 0; i < ninst; ++i) {
+      const dmf_csr& gi = graphs[i];
+      if (gi.n < 2 || gi.m < 0 || s[i] < 0 || s[i] >= gi.n || t[i] < 0 || t[i] >= gi.n || s[i] == t[i])
+        return DMF_EINVAL;
+      if (const int rc = dmf_csr_check(&gi)) return rc;
+      voff[i + 1] = voff[i] + gi.n;
+      aoff[i + 1] = aoff[i] + gi.m;
     }
-    for (int32_t e = 0; e < gi.m; ++e) {
-      col[ao + e] = vo + gi.col[e];
-      cap[ao + e] = gi.cap[e];
-      rev[ao + e] = ao + gi.rev[e];
+    if (voff[ninst] > INT32_MAX - 1 || aoff[ninst] > INT32_MAX) return DMF_EINVAL;
+    const int32_t n = static_cast<int32_t>(voff[ninst]), m = static_cast<int32_t>(aoff[ninst]);
+    std::vector<int32_t> row(n + 1), col(m), cap(m), rev(m), inst(n), srcs(ninst), snks(ninst);
+    for (int32_t i = 0; i < ninst; ++i) {
+      const dmf_csr& gi = graphs[i];
+      const int32_t vo = static_cast<int32_t>(voff[i]), ao = static_cast<int32_t>(aoff[i]);
+      for (int32_t v = 0; v < gi.n; ++v) {
+        row[vo + v] = ao + gi.row[v];
+        inst[vo + v] = i;
+      }
+      for (int32_t e = 0; e < gi.m; ++e) {
+        col[ao + e] = vo + gi.col[e];
+        cap[ao + e] = gi.cap[e];
+        rev[ao + e] = ao + gi.rev[e];
+      }
+      srcs[i] = vo + s[i];
+      snks[i] = vo + t[i];
     }
-    srcs[i] = vo + s[i];
-    snks[i] = vo + t[i];
-  }
-  row[n] = m;
-  dmf_csr u{n, m, row.data(), col.data(), cap.data(), rev.data()};
-  int rc = create_engine(&u, srcs, snks, &inst, ninst, out);
-  if (rc) return rc;
-  (*out)->voff.assign(voff.begin(), voff.end());
-  return DMF_OK;
+    row[n] = m;
+    dmf_csr u{n, m, row.data(), col.data(), cap.data(), rev.data()};
+    int rc = create_engine(&u, srcs, snks, &inst, ninst, out);
+    if (rc) return rc;
+    (*out)->voff.assign(voff.begin(), voff.end());
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_destroy(dmf_engine* E) {
-  if (!E) return DMF_EINVAL;
-  if (E->stream) hipStreamSynchronize(E->stream);
-  Dev& D = E->D;
-  hipFree(const_cast<int32_t*>(D.row));
-  hipFree(const_cast<int32_t*>(D.col));
-  hipFree(const_cast<int32_t*>(D.rev));
-  hipFree(D.cap);
-  hipFree(E->d_cap0);
-  hipFree(D.res);
-  hipFree(D.rres);
-  hipFree(D.excess);
-  hipFree(D.height);
-  hipFree(D.order);
-  hipFree(D.wl);
-  hipFree(D.wl2);
-  hipFree(D.part);
-  hipFree(D.side);
-  hipFree(D.pfinal);
-  hipFree(D.ctrl);
-  hipFree(D.hkey);
-  hipFree(D.hval);
-  hipFree(D.owner);
-  hipFree(D.ctrs);
-  hipFree(const_cast<uint8_t*>(D.term));
-  hipFree(const_cast<int32_t*>(D.srcs));
-  hipFree(const_cast<int32_t*>(D.snks));
-  hipFree(const_cast<int32_t*>(D.inst));
-  hipFree(D.iflow);
-  hipFree(D.lev16);
-  hipFree(const_cast<uint32_t*>(D.widebits));
-  hipFree(D.wq_vert);
-  hipFree(D.wq_base);
-  hipFree(const_cast<int32_t*>(D.wide_vert));
-  hipFree(const_cast<int32_t*>(D.wide_base));
-  hipFree(const_cast<int32_t*>(D.wave_vert));
-  hipFree(const_cast<int32_t*>(D.wave_base));
-  hipFree(E->d_usrc);
-  hipFree(E->d_udst);
-  hipFree(E->d_uw);
-  hipFree(E->d_uarc);
-  hipFree(E->d_vmap);
-  destroy_segments(E);
-  hipFree(E->d_dev);
-  if (E->h_ctrl) hipHostFree(E->h_ctrl);
-  if (E->h_sched) hipHostFree(E->h_sched);
-  hipFree(E->d_sched_w);
-  hipFree(E->d_sched);
-  if (E->trace) std::fclose(E->trace);
-  if (E->ev0) hipEventDestroy(E->ev0);
-  if (E->pev0) (void)hipEventDestroy(E->pev0);
-  if (E->pev1) (void)hipEventDestroy(E->pev1);
-  if (E->ev1) hipEventDestroy(E->ev1);
-  if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
-  delete E;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    if (E->stream) hipStreamSynchronize(E->stream);
+    Dev& D = E->D;
+    hipFree(const_cast<int32_t*>(D.row));
+    hipFree(const_cast<int32_t*>(D.col));
+    hipFree(const_cast<int32_t*>(D.rev));
+    hipFree(D.cap);
+    hipFree(E->d_cap0);
+    hipFree(D.res);
+    hipFree(D.rres);
+    hipFree(D.excess);
+    hipFree(D.height);
+    hipFree(D.order);
+    hipFree(D.wl);
+    hipFree(D.wl2);
+    hipFree(D.part);
+    hipFree(D.side);
+    hipFree(D.pfinal);
+    hipFree(D.ctrl);
+    hipFree(D.hkey);
+    hipFree(D.hval);
+    hipFree(D.owner);
+    hipFree(D.ctrs);
+    hipFree(const_cast<uint8_t*>(D.term));
+    hipFree(const_cast<int32_t*>(D.srcs));
+    hipFree(const_cast<int32_t*>(D.snks));
+    hipFree(const_cast<int32_t*>(D.inst));
+    hipFree(D.iflow);
+    hipFree(D.lev16);
+    hipFree(const_cast<uint32_t*>(D.widebits));
+    hipFree(D.wq_vert);
+    hipFree(D.wq_base);
+    hipFree(const_cast<int32_t*>(D.wide_vert));
+    hipFree(const_cast<int32_t*>(D.wide_base));
+    hipFree(const_cast<int32_t*>(D.wave_vert));
+    hipFree(const_cast<int32_t*>(D.wave_base));
+    hipFree(E->d_usrc);
+    hipFree(E->d_udst);
+    hipFree(E->d_uw);
+    hipFree(E->d_uarc);
+    hipFree(E->d_vmap);
+    destroy_segments(E);
+    hipFree(E->d_dev);
+    if (E->h_ctrl) hipHostFree(E->h_ctrl);
+    if (E->h_sched) hipHostFree(E->h_sched);
+    hipFree(E->d_sched_w);
+    hipFree(E->d_sched);
+    if (E->trace) std::fclose(E->trace);
+    if (E->ev0) hipEventDestroy(E->ev0);
+    if (E->pev0) (void)hipEventDestroy(E->pev0);
+    if (E->pev1) (void)hipEventDestroy(E->pev1);
+    if (E->ev1) hipEventDestroy(E->ev1);
+    if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
+    delete E;
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_set_stream(dmf_engine* E, void* hip_stream) {
-  if (!E) return DMF_EINVAL;
-  HIP_OK(hipStreamSynchronize(E->stream));
-  if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
-  if (hip_stream) {
-    E->stream = static_cast<hipStream_t>(hip_stream);
-    E->own_stream = false;
-  } else {
-    HIP_OK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
-    E->own_stream = true;
-  }
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    HIP_OK(hipStreamSynchronize(E->stream));
+    if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
+    if (hip_stream) {
+      E->stream = static_cast<hipStream_t>(hip_stream);
+      E->own_stream = false;
+    } else {
+      HIP_OK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+      E->own_stream = true;
+    }
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_reset(dmf_engine* E) {
-  if (!E) return DMF_EINVAL;
-  Dev& D = E->D;
-  HIP_OK(hipMemcpyAsync(D.cap, E->d_cap0, sizeof(int32_t) * E->m, hipMemcpyDeviceToDevice, E->stream));
-  HIP_OK(hipMemsetAsync(D.res, 0, sizeof(int32_t) * E->m, E->stream));
-  HIP_OK(hipMemsetAsync(D.rres, 0, sizeof(int32_t) * E->m, E->stream));
-  HIP_OK(hipMemsetAsync(D.excess, 0, sizeof(long long) * E->n, E->stream));
-  HIP_OK(hipMemsetAsync(D.side, 0, E->n, E->stream));
-  HIP_OK(hipStreamSynchronize(E->stream));
-  E->pp_ready = false;
-  E->labels_fresh = false;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    Dev& D = E->D;
+    HIP_OK(hipMemcpyAsync(D.cap, E->d_cap0, sizeof(int32_t) * E->m, hipMemcpyDeviceToDevice, E->stream));
+    HIP_OK(hipMemsetAsync(D.res, 0, sizeof(int32_t) * E->m, E->stream));
+    HIP_OK(hipMemsetAsync(D.rres, 0, sizeof(int32_t) * E->m, E->stream));
+    HIP_OK(hipMemsetAsync(D.excess, 0, sizeof(long long) * E->n, E->stream));
+    HIP_OK(hipMemsetAsync(D.side, 0, E->n, E->stream));
+    HIP_OK(hipStreamSynchronize(E->stream));
+    E->pp_ready = false;
+    E->labels_fresh = false;
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_static_solve(dmf_engine* E, const dmf_params* p, int64_t* flow) {
-  if (!E) return DMF_EINVAL;
-  E->D.k = 0;
-  return run_solve(E, p, 0, flow);
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    E->D.k = 0;
+    return run_solve(E, p, 0, flow);
+  });
 }
 
 int dmf_engine_apply_batch_device(dmf_engine* E, int32_t k, const int32_t* d_src,
                                   const int32_t* d_dst, const int32_t* d_w, const dmf_params* p,
                                   int64_t* flow) {
-  if (!E || k < 0 || (k > 0 && (!d_src || !d_dst || !d_w))) return DMF_EINVAL;
-  int rc = ensure_update_capacity(E, std::max(k, 1));
-  if (rc) return rc;
-  E->D.usrc = d_src;
-  E->D.udst = d_dst;
-  E->D.uw = d_w;
-  E->D.uarc = E->d_uarc;
-  E->D.k = k;
-  rc = run_solve(E, p, 1, flow);
-  E->D.k = 0;
-  return rc;
+  return guarded([&]() -> int {
+    if (!E || k < 0 || (k > 0 && (!d_src || !d_dst || !d_w))) return DMF_EINVAL;
+    int rc = ensure_update_capacity(E, std::max(k, 1));
+    if (rc) return rc;
+    E->D.usrc = d_src;
+    E->D.udst = d_dst;
+    E->D.uw = d_w;
+    E->D.uarc = E->d_uarc;
+    E->D.k = k;
+    rc = run_solve(E, p, 1, flow);
+    E->D.k = 0;
+    return rc;
+  });
 }
 
 int dmf_engine_apply_batch(dmf_engine* E, int32_t k, const int32_t* src, const int32_t* dst,
                            const int32_t* w, const dmf_params* p, int64_t* flow) {
-  if (!E || k < 0 || (k > 0 && (!src || !dst || !w))) return DMF_EINVAL;
-  int rc = ensure_update_capacity(E, std::max(k, 1));
-  if (rc) return rc;
-  if (k > 0) {
-    HIP_OK(hipMemcpyAsync(E->d_usrc, src, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
-    HIP_OK(hipMemcpyAsync(E->d_udst, dst, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
-    HIP_OK(hipMemcpyAsync(E->d_uw, w, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
-  }
-  return dmf_engine_apply_batch_device(E, k, E->d_usrc, E->d_udst, E->d_uw, p, flow);
+  return guarded([&]() -> int {
+    if (!E || k < 0 || (k > 0 && (!src || !dst || !w))) return DMF_EINVAL;
+    int rc = ensure_update_capacity(E, std::max(k, 1));
+    if (rc) return rc;
+    if (k > 0) {
+      HIP_OK(hipMemcpyAsync(E->d_usrc, src, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
+      HIP_OK(hipMemcpyAsync(E->d_udst, dst, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
+      HIP_OK(hipMemcpyAsync(E->d_uw, w, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream));
+    }
+    return dmf_engine_apply_batch_device(E, k, E->d_usrc, E->d_udst, E->d_uw, p, flow);
+  });
 }
 
 int dmf_engine_flow(dmf_engine* E, int64_t* flow) {
-  if (!E || !flow) return DMF_EINVAL;
-  HIP_OK(hipMemsetAsync(&E->D.ctrl->flow, 0, sizeof(long long), E->stream));
-  HIP_OK(hipMemsetAsync(E->D.iflow, 0, sizeof(long long) * E->D.ninst, E->stream));
-  hipLaunchKernelGGL(k_flow, dim3(blocks_for(E->n)), dim3(kBlock), 0, E->stream, E->D);
-  HIP_OK(hipGetLastError());
-  int rc = sync_ctrl(E);
-  if (rc) return rc;
-  *flow = E->h_ctrl->flow;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E || !flow) return DMF_EINVAL;
+    HIP_OK(hipMemsetAsync(&E->D.ctrl->flow, 0, sizeof(long long), E->stream));
+    HIP_OK(hipMemsetAsync(E->D.iflow, 0, sizeof(long long) * E->D.ninst, E->stream));
+    hipLaunchKernelGGL(k_flow, dim3(blocks_for(E->n)), dim3(kBlock), 0, E->stream, E->D);
+    HIP_OK(hipGetLastError());
+    int rc = sync_ctrl(E);
+    if (rc) return rc;
+    *flow = E->h_ctrl->flow;
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_min_cut(dmf_engine* E, uint8_t* side) {
-  if (!E || !side) return DMF_EINVAL;
-  Dev& D = E->D;
-  int32_t* mark = D.wl2;  // scratch between solves
-  int rc = DMF_OK;
-  do {
-    if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) break;
-    LAUNCH(k_cut_init, blocks_for(E->n), D, mark);
-    int lo = 0, lvl = 0;
-    for (;;) {
-      int32_t cnt = 0;
-      if ((rc = read_int(E, &D.ctrl->qcnt[lvl % 3], &cnt))) break;
-      if (cnt == 0) break;
-      if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) break;
-      LAUNCH(k_cut_level, blocks_for(cnt), D, mark, lo, cnt, (lvl + 1) % 3);
-      lo += cnt;
-      ++lvl;
-    }
-    if (rc) break;
-    std::vector<int32_t> h(E->n);
-    if (hipMemcpyAsync(h.data(), mark, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream) != hipSuccess ||
-        hipStreamSynchronize(E->stream) != hipSuccess) {
-      rc = DMF_EDEVICE;
-      break;
-    }
-    if (E->h_vmap.empty()) {
-      for (int32_t v = 0; v < E->n; ++v) side[v] = h[v] ? 1 : 0;
-    } else {
-      for (int32_t v = 0; v < E->n; ++v) side[v] = h[E->h_vmap[v]] ? 1 : 0;
-    }
-  } while (false);
-  return rc;
+  return guarded([&]() -> int {
+    if (!E || !side) return DMF_EINVAL;
+    Dev& D = E->D;
+    int32_t* mark = D.wl2;  // scratch between solves
+    int rc = DMF_OK;
+    do {
+      if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) break;
+      LAUNCH(k_cut_init, blocks_for(E->n), D, mark);
+      int lo = 0, lvl = 0;
+      for (;;) {
+        int32_t cnt = 0;
+        if ((rc = read_int(E, &D.ctrl->qcnt[lvl % 3], &cnt))) break;
+        if (cnt == 0) break;
+        if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) break;
+        LAUNCH(k_cut_level, blocks_for(cnt), D, mark, lo, cnt, (lvl + 1) % 3);
+        lo += cnt;
+        ++lvl;
+      }
+      if (rc) break;
+      std::vector<int32_t> h(E->n);
+      if (hipMemcpyAsync(h.data(), mark, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream) != hipSuccess ||
+          hipStreamSynchronize(E->stream) != hipSuccess) {
+        rc = DMF_EDEVICE;
+        break;
+      }
+      if (E->h_vmap.empty()) {
+        for (int32_t v = 0; v < E->n; ++v) side[v] = h[v] ? 1 : 0;
+      } else {
+        for (int32_t v = 0; v < E->n; ++v) side[v] = h[E->h_vmap[v]] ? 1 : 0;
+      }
+    } while (false);
+    return rc;
+  });
 }
 
 int dmf_engine_read_state(dmf_engine* E, int32_t* res, int32_t* cap, int64_t* excess,
                           int32_t* height) {
-  if (!E) return DMF_EINVAL;
-  if (!E->h_vmap.empty()) return read_state_scattered(E, res, cap, excess, height);
-  if (res) HIP_OK(hipMemcpyAsync(res, E->D.res, sizeof(int32_t) * E->m, hipMemcpyDeviceToHost, E->stream));
-  if (cap) HIP_OK(hipMemcpyAsync(cap, E->D.cap, sizeof(int32_t) * E->m, hipMemcpyDeviceToHost, E->stream));
-  if (excess)
-    HIP_OK(hipMemcpyAsync(excess, E->D.excess, sizeof(long long) * E->n, hipMemcpyDeviceToHost, E->stream));
-  if (height)
-    HIP_OK(hipMemcpyAsync(height, E->D.height, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream));
-  HIP_OK(hipStreamSynchronize(E->stream));
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    if (!E->h_vmap.empty()) return read_state_scattered(E, res, cap, excess, height);
+    if (res) HIP_OK(hipMemcpyAsync(res, E->D.res, sizeof(int32_t) * E->m, hipMemcpyDeviceToHost, E->stream));
+    if (cap) HIP_OK(hipMemcpyAsync(cap, E->D.cap, sizeof(int32_t) * E->m, hipMemcpyDeviceToHost, E->stream));
+    if (excess)
+      HIP_OK(hipMemcpyAsync(excess, E->D.excess, sizeof(long long) * E->n, hipMemcpyDeviceToHost, E->stream));
+    if (height)
+      HIP_OK(hipMemcpyAsync(height, E->D.height, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream));
+    HIP_OK(hipStreamSynchronize(E->stream));
+    return DMF_OK;
+  });
 }
 
 int dmf_engine_counters(dmf_engine* E, dmf_counters* c) {
-  if (!E || !c) return DMF_EINVAL;
-  *c = E->last;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E || !c) return DMF_EINVAL;
+    *c = E->last;
+    return DMF_OK;
+  });
 }
 
 int64_t dmf_engine_device_bytes(const dmf_engine* E) { return E ? E->bytes : 0; }
 
 int dmf_engine_set_profiling(dmf_engine* E, int on) {
-  if (!E) return DMF_EINVAL;
-  if (on && !E->pev0) {
-    HIP_OK(hipEventCreate(&E->pev0));
-    HIP_OK(hipEventCreate(&E->pev1));
-  }
-  E->profile = on != 0;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    if (on && !E->pev0) {
+      HIP_OK(hipEventCreate(&E->pev0));
+      HIP_OK(hipEventCreate(&E->pev1));
+    }
+    E->profile = on != 0;
+    return DMF_OK;
+  });
 }
 
 int64_t dmf_engine_footprint(int32_t n, int32_t m) {
@@ -1367,144 +1396,158 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
 }
 
 int dmf_engine_reserve_updates(dmf_engine* E, int32_t k) {
-  if (!E || k < 0) return DMF_EINVAL;
-  return ensure_update_capacity(E, std::max(k, 1));
+  return guarded([&]() -> int {
+    if (!E || k < 0) return DMF_EINVAL;
+    return ensure_update_capacity(E, std::max(k, 1));
+  });
 }
 
 int32_t dmf_engine_instances(const dmf_engine* E) { return E ? E->ninst : 0; }
 
 int dmf_batch_flows(dmf_engine* E, int64_t* flows) {
-  if (!E || !flows) return DMF_EINVAL;
-  int64_t total = 0;
-  int rc = dmf_engine_flow(E, &total);
-  if (rc) return rc;
-  HIP_OK(hipMemcpy(flows, E->D.iflow, sizeof(int64_t) * E->ninst, hipMemcpyDeviceToHost));
-  if (E->ninst == 1) flows[0] = total;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!E || !flows) return DMF_EINVAL;
+    int64_t total = 0;
+    int rc = dmf_engine_flow(E, &total);
+    if (rc) return rc;
+    HIP_OK(hipMemcpy(flows, E->D.iflow, sizeof(int64_t) * E->ninst, hipMemcpyDeviceToHost));
+    if (E->ninst == 1) flows[0] = total;
+    return DMF_OK;
+  });
 }
 
 int dmf_batch_static_solve(dmf_engine* E, const dmf_params* p, int64_t* flows) {
-  if (!E || !flows) return DMF_EINVAL;
-  int64_t total = 0;
-  int rc = dmf_engine_static_solve(E, p, &total);
-  if (rc) return rc;
-  return dmf_batch_flows(E, flows);
+  return guarded([&]() -> int {
+    if (!E || !flows) return DMF_EINVAL;
+    int64_t total = 0;
+    int rc = dmf_engine_static_solve(E, p, &total);
+    if (rc) return rc;
+    return dmf_batch_flows(E, flows);
+  });
 }
 
 int dmf_batch_apply(dmf_engine* E, const int32_t* counts, const int32_t* src, const int32_t* dst,
                     const int32_t* w, const dmf_params* p, int64_t* flows) {
-  if (!E || !counts || !flows) return DMF_EINVAL;
-  if (E->voff.empty()) E->voff = {0, E->n};
-  int64_t total = 0;
-  for (int32_t i = 0; i < E->ninst; ++i) {
-    if (counts[i] < 0) return DMF_EINVAL;
-    total += counts[i];
-  }
-  if (total > INT32_MAX || (total > 0 && (!src || !dst || !w))) return DMF_EINVAL;
-  std::vector<int32_t> gs(total), gd(total);
-  int64_t j = 0;
-  for (int32_t i = 0; i < E->ninst; ++i) {
-    const int64_t nv = E->voff[i + 1] - E->voff[i];
-    for (int32_t q = 0; q < counts[i]; ++q, ++j) {
-      if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv) return DMF_EINVAL;
-      gs[j] = static_cast<int32_t>(E->voff[i] + src[j]);
-      gd[j] = static_cast<int32_t>(E->voff[i] + dst[j]);
+  return guarded([&]() -> int {
+    if (!E || !counts || !flows) return DMF_EINVAL;
+    if (E->voff.empty()) E->voff = {0, E->n};
+    int64_t total = 0;
+    for (int32_t i = 0; i < E->ninst; ++i) {
+      if (counts[i] < 0) return DMF_EINVAL;
+      total += counts[i];
     }
-  }
-  int64_t f = 0;
-  int rc = dmf_engine_apply_batch(E, static_cast<int32_t>(total), gs.data(), gd.data(), w, p, &f);
-  if (rc) return rc;
-  return dmf_batch_flows(E, flows);
+    if (total > INT32_MAX || (total > 0 && (!src || !dst || !w))) return DMF_EINVAL;
+    std::vector<int32_t> gs(total), gd(total);
+    int64_t j = 0;
+    for (int32_t i = 0; i < E->ninst; ++i) {
+      const int64_t nv = E->voff[i + 1] - E->voff[i];
+      for (int32_t q = 0; q < counts[i]; ++q, ++j) {
+        if (src[j] < 0 || src[j] >= nv || dst[j] < 0 || dst[j] >= nv) return DMF_EINVAL;
+        gs[j] = static_cast<int32_t>(E->voff[i] + src[j]);
+        gd[j] = static_cast<int32_t>(E->voff[i] + dst[j]);
+      }
+    }
+    int64_t f = 0;
+    int rc = dmf_engine_apply_batch(E, static_cast<int32_t>(total), gs.data(), gd.data(), w, p, &f);
+    if (rc) return rc;
+    return dmf_batch_flows(E, flows);
+  });
 }
 
 int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, int32_t* wide_t) {
-  if (!E) return DMF_EINVAL;
-  static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {16, 64}};
-  double best_ms = 1e300;
-  int best = 1;
-  for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {
-    int rc = setup_wide(E, cand[c][0], cand[c][1]);
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {16, 64}};
+    double best_ms = 1e300;
+    int best = 1;
+    for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {
+      int rc = setup_wide(E, cand[c][0], cand[c][1]);
+      if (!rc) rc = dmf_engine_reset(E);
+      int64_t f = 0;
+      double ms = 0;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {  // first solve warms caches
+        rc = dmf_engine_static_solve(E, p, &f);
+        ms = E->last.device_ms;
+      }
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    int rc = setup_wide(E, cand[best][0], cand[best][1]);
     if (!rc) rc = dmf_engine_reset(E);
-    int64_t f = 0;
-    double ms = 0;
-    for (int rep = 0; rep < 2 && !rc; ++rep) {  // first solve warms caches
-      rc = dmf_engine_static_solve(E, p, &f);
-      ms = E->last.device_ms;
-    }
-    if (rc) return rc;
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = c;
-    }
-  }
-  int rc = setup_wide(E, cand[best][0], cand[best][1]);
-  if (!rc) rc = dmf_engine_reset(E);
-  if (lane_max) *lane_max = cand[best][0];
-  if (wide_t) *wide_t = cand[best][1];
-  return rc;
+    if (lane_max) *lane_max = cand[best][0];
+    if (wide_t) *wide_t = cand[best][1];
+    return rc;
+  });
 }
 
 int dmf_recalculate_max_flow(const dmf_csr* g, int32_t k, const int32_t* src, const int32_t* dst,
                              const int32_t* w, int32_t s, int32_t t, int32_t kc, int32_t variant,
                              int64_t* flow, double* ms_static, double* ms_batch) {
-  dmf_engine* E = nullptr;
-  int rc = dmf_engine_create(g, s, t, &E);
-  if (rc) return rc;
-  dmf_params p;
-  dmf_params_default(g, &p);
-  p.kc = kc > 0 ? kc : p.kc;
-  p.variant = variant;
-  int64_t f = 0;
-  rc = dmf_engine_static_solve(E, &p, &f);
-  if (!rc && ms_static) *ms_static = E->last.device_ms;
-  if (!rc) rc = dmf_engine_apply_batch(E, k, src, dst, w, &p, &f);
-  if (!rc && ms_batch) *ms_batch = E->last.device_ms;
-  if (!rc && flow) *flow = f;
-  dmf_engine_destroy(E);
-  return rc;
+  return guarded([&]() -> int {
+    dmf_engine* E = nullptr;
+    int rc = dmf_engine_create(g, s, t, &E);
+    if (rc) return rc;
+    dmf_params p;
+    dmf_params_default(g, &p);
+    p.kc = kc > 0 ? kc : p.kc;
+    p.variant = variant;
+    int64_t f = 0;
+    rc = dmf_engine_static_solve(E, &p, &f);
+    if (!rc && ms_static) *ms_static = E->last.device_ms;
+    if (!rc) rc = dmf_engine_apply_batch(E, k, src, dst, w, &p, &f);
+    if (!rc && ms_batch) *ms_batch = E->last.device_ms;
+    if (!rc && flow) *flow = f;
+    dmf_engine_destroy(E);
+    return rc;
+  });
 }
 
 int dmf_static_recalculate(const dmf_csr* g, int32_t k, const int32_t* src, const int32_t* dst,
                            const int32_t* w, int32_t s, int32_t t, int32_t kc, int32_t variant,
                            int64_t* flow, double* ms) {
-  dmf_engine* E = nullptr;
-  int rc = dmf_engine_create(g, s, t, &E);
-  if (rc) return rc;
-  // update_kernel (code/dyn_data_bench.cu:300-322): set capacities, then solve.
-  rc = ensure_update_capacity(E, std::max(k, 1));
-  if (!rc && k > 0) {
-    Dev& D = E->D;
-    bool ok = hipMemcpyAsync(E->d_usrc, src, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-              hipMemcpyAsync(E->d_udst, dst, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-              hipMemcpyAsync(E->d_uw, w, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
-              hipMemsetAsync(D.ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess;
-    if (!ok) rc = DMF_EDEVICE;
-    if (!rc) {
-      D.usrc = E->d_usrc;
-      D.udst = E->d_udst;
-      D.uw = E->d_uw;
-      D.uarc = E->d_uarc;
-      D.k = k;
-      hipLaunchKernelGGL(k_upd_lookup, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D);
-      hipLaunchKernelGGL(k_upd_apply, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D, 1, 0);
-      hipLaunchKernelGGL(k_upd_release, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D);
-      if (hipGetLastError() != hipSuccess) rc = DMF_EDEVICE;
-      if (!rc) rc = sync_ctrl(E);
-      if (!rc && E->h_ctrl->missing) rc = DMF_ENOARC;
-      if (!rc && E->h_ctrl->invalid) rc = DMF_EINVAL;
-      D.k = 0;
+  return guarded([&]() -> int {
+    dmf_engine* E = nullptr;
+    int rc = dmf_engine_create(g, s, t, &E);
+    if (rc) return rc;
+    // update_kernel (code/dyn_data_bench.cu:300-322): set capacities, then solve.
+    rc = ensure_update_capacity(E, std::max(k, 1));
+    if (!rc && k > 0) {
+      Dev& D = E->D;
+      bool ok = hipMemcpyAsync(E->d_usrc, src, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+                hipMemcpyAsync(E->d_udst, dst, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+                hipMemcpyAsync(E->d_uw, w, sizeof(int32_t) * k, hipMemcpyHostToDevice, E->stream) == hipSuccess &&
+                hipMemsetAsync(D.ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess;
+      if (!ok) rc = DMF_EDEVICE;
+      if (!rc) {
+        D.usrc = E->d_usrc;
+        D.udst = E->d_udst;
+        D.uw = E->d_uw;
+        D.uarc = E->d_uarc;
+        D.k = k;
+        hipLaunchKernelGGL(k_upd_lookup, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D);
+        hipLaunchKernelGGL(k_upd_apply, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D, 1, 0);
+        hipLaunchKernelGGL(k_upd_release, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D);
+        if (hipGetLastError() != hipSuccess) rc = DMF_EDEVICE;
+        if (!rc) rc = sync_ctrl(E);
+        if (!rc && E->h_ctrl->missing) rc = DMF_ENOARC;
+        if (!rc && E->h_ctrl->invalid) rc = DMF_EINVAL;
+        D.k = 0;
+      }
     }
-  }
-  dmf_params p;
-  dmf_params_default(g, &p);
-  p.kc = kc > 0 ? kc : p.kc;
-  p.variant = variant;
-  int64_t f = 0;
-  if (!rc) rc = dmf_engine_static_solve(E, &p, &f);
-  if (!rc && ms) *ms = E->last.device_ms;
-  if (!rc && flow) *flow = f;
-  dmf_engine_destroy(E);
-  return rc;
+    dmf_params p;
+    dmf_params_default(g, &p);
+    p.kc = kc > 0 ? kc : p.kc;
+    p.variant = variant;
+    int64_t f = 0;
+    if (!rc) rc = dmf_engine_static_solve(E, &p, &f);
+    if (!rc && ms) *ms = E->last.device_ms;
+    if (!rc && flow) *flow = f;
+    dmf_engine_destroy(E);
+    return rc;
+  });
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../../include/dmf.h"
+#include "abi_guard.hpp"
 
 namespace {
 
@@ -248,189 +249,203 @@ void dmf_server_config_default(dmf_server_config* c) {
 }
 
 int dmf_server_create(const dmf_server_config* cfg, dmf_server** out) {
-  if (!out) return DMF_EINVAL;
-  *out = nullptr;
-  if (dmf_device_count() < 1) return DMF_EDEVICE;
-  auto S = std::make_unique<dmf_server>();
-  if (cfg) S->cfg = *cfg;
-  else dmf_server_config_default(&S->cfg);
-  if (S->cfg.max_sessions <= 0) S->cfg.max_sessions = 64;
-  if (S->cfg.hbm_budget < 0 || S->cfg.staging_bytes < 0) return DMF_EINVAL;
-  if (hipGetDevice(&S->device) != hipSuccess) return DMF_EDEVICE;
-  size_t free_b = 0, total_b = 0;
-  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return DMF_EDEVICE;
-  if (S->cfg.hbm_budget == 0) S->cfg.hbm_budget = static_cast<int64_t>(free_b / 10 * 9);
-  if (S->cfg.staging_bytes == 0) S->cfg.staging_bytes = std::max<int64_t>(S->cfg.hbm_budget / 8, 1 << 20);
-  S->cfg.staging_bytes = align256(S->cfg.staging_bytes);
-  if (S->cfg.staging_bytes > S->cfg.hbm_budget) return DMF_EADMIT;
-  if (hipMalloc(reinterpret_cast<void**>(&S->d_arena), S->cfg.staging_bytes) != hipSuccess) return DMF_ENOMEM;
-  if (hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess) {
-    (void)hipFree(S->d_arena);
-    return DMF_EDEVICE;
-  }
-  S->arena.init(S->cfg.staging_bytes);
-  S->held = S->cfg.staging_bytes;
-  S->t0 = std::chrono::steady_clock::now();
-  try {
-    S->stager = std::thread(stager_loop, S.get());
-  } catch (...) {  // no thread available; nothing may escape the C-ABI
-    (void)hipStreamDestroy(S->copy_stream);
-    (void)hipFree(S->d_arena);
-    return DMF_ENOMEM;
-  }
-  *out = S.release();
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!out) return DMF_EINVAL;
+    *out = nullptr;
+    if (dmf_device_count() < 1) return DMF_EDEVICE;
+    auto S = std::make_unique<dmf_server>();
+    if (cfg) S->cfg = *cfg;
+    else dmf_server_config_default(&S->cfg);
+    if (S->cfg.max_sessions <= 0) S->cfg.max_sessions = 64;
+    if (S->cfg.hbm_budget < 0 || S->cfg.staging_bytes < 0) return DMF_EINVAL;
+    if (hipGetDevice(&S->device) != hipSuccess) return DMF_EDEVICE;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return DMF_EDEVICE;
+    if (S->cfg.hbm_budget == 0) S->cfg.hbm_budget = static_cast<int64_t>(free_b / 10 * 9);
+    if (S->cfg.staging_bytes == 0) S->cfg.staging_bytes = std::max<int64_t>(S->cfg.hbm_budget / 8, 1 << 20);
+    S->cfg.staging_bytes = align256(S->cfg.staging_bytes);
+    if (S->cfg.staging_bytes > S->cfg.hbm_budget) return DMF_EADMIT;
+    if (hipMalloc(reinterpret_cast<void**>(&S->d_arena), S->cfg.staging_bytes) != hipSuccess) return DMF_ENOMEM;
+    if (hipStreamCreateWithFlags(&S->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipFree(S->d_arena);
+      return DMF_EDEVICE;
+    }
+    S->arena.init(S->cfg.staging_bytes);
+    S->held = S->cfg.staging_bytes;
+    S->t0 = std::chrono::steady_clock::now();
+    try {
+      S->stager = std::thread(stager_loop, S.get());
+    } catch (...) {  // no thread available; nothing may escape the C-ABI
+      (void)hipStreamDestroy(S->copy_stream);
+      (void)hipFree(S->d_arena);
+      return DMF_ENOMEM;
+    }
+    *out = S.release();
+    return DMF_OK;
+  });
 }
 
 int dmf_server_open(dmf_server* S, const dmf_csr* g, int32_t s, int32_t t, int32_t max_batch,
                     int32_t* session, int64_t* static_flow) {
-  if (!S || !g || !session || max_batch < 1) return DMF_EINVAL;
-  const int64_t need = dmf_engine_footprint(g->n, g->m) + static_cast<int64_t>(max_batch) * 16;
-  {
+  return guarded([&]() -> int {
+    if (!S || !g || !session || max_batch < 1) return DMF_EINVAL;
+    const int64_t need = dmf_engine_footprint(g->n, g->m) + static_cast<int64_t>(max_batch) * 16;
+    auto ss = std::make_unique<Session>();  // before the reservation, which nothing below throws past
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (static_cast<int32_t>(S->sessions.size()) >= S->cfg.max_sessions) return DMF_EADMIT;
+      if (S->held + need > S->cfg.hbm_budget) return DMF_EADMIT;
+      S->held += need;  // reserved while the engine is built
+    }
+    ss->params = S->cfg.params;
+    if (ss->params.kc <= 0) {
+      dmf_params d;
+      dmf_params_default(g, &d);
+      ss->params.kc = d.kc;
+    }
+    int rc = dmf_engine_create(g, s, t, &ss->eng);
+    if (!rc) rc = dmf_engine_reserve_updates(ss->eng, max_batch);
+    int64_t f = 0;
+    if (!rc) {
+      dmf_params p = ss->params;
+      p.scheduler = DMF_SCHED_FIXED;  // the static solve has no batch to schedule
+      p.variant = DMF_VARIANT_PP;     // a push-pull static state lets every variant take the batches
+      rc = dmf_engine_static_solve(ss->eng, &p, &f);
+    }
     std::lock_guard<std::mutex> lk(S->mu);
-    if (static_cast<int32_t>(S->sessions.size()) >= S->cfg.max_sessions) return DMF_EADMIT;
-    if (S->held + need > S->cfg.hbm_budget) return DMF_EADMIT;
-    S->held += need;  // reserved while the engine is built
-  }
-  auto ss = std::make_unique<Session>();
-  ss->params = S->cfg.params;
-  if (ss->params.kc <= 0) {
-    dmf_params d;
-    dmf_params_default(g, &d);
-    ss->params.kc = d.kc;
-  }
-  int rc = dmf_engine_create(g, s, t, &ss->eng);
-  if (!rc) rc = dmf_engine_reserve_updates(ss->eng, max_batch);
-  int64_t f = 0;
-  if (!rc) {
-    dmf_params p = ss->params;
-    p.scheduler = DMF_SCHED_FIXED;  // the static solve has no batch to schedule
-    p.variant = DMF_VARIANT_PP;     // a push-pull static state lets every variant take the batches
-    rc = dmf_engine_static_solve(ss->eng, &p, &f);
-  }
-  std::lock_guard<std::mutex> lk(S->mu);
-  S->held -= need;
-  if (rc) {
-    if (ss->eng) dmf_engine_destroy(ss->eng);
-    return rc;
-  }
-  ss->bytes = dmf_engine_device_bytes(ss->eng);
-  S->held += ss->bytes;
-  ss->max_batch = max_batch;
-  ss->id = S->next_session++;
-  Session* raw = ss.get();
-  S->sessions[ss->id] = std::move(ss);
-  try {
-    raw->solver = std::thread(solver_loop, S, raw);
-  } catch (...) {  // no thread available: undo the session
-    S->held -= raw->bytes;
-    dmf_engine_destroy(raw->eng);
-    S->sessions.erase(raw->id);
-    return DMF_ENOMEM;
-  }
-  *session = raw->id;
-  if (static_flow) *static_flow = f;
-  return DMF_OK;
+    S->held -= need;
+    if (rc) {
+      if (ss->eng) dmf_engine_destroy(ss->eng);
+      return rc;
+    }
+    ss->bytes = dmf_engine_device_bytes(ss->eng);
+    S->held += ss->bytes;
+    ss->max_batch = max_batch;
+    ss->id = S->next_session++;
+    Session* raw = ss.get();
+    try {
+      S->sessions[raw->id] = std::move(ss);
+      raw->solver = std::thread(solver_loop, S, raw);
+    } catch (...) {  // no memory or thread: undo the session (ss frees it if never inserted)
+      S->held -= raw->bytes;
+      dmf_engine_destroy(raw->eng);
+      S->sessions.erase(raw->id);
+      return DMF_ENOMEM;
+    }
+    *session = raw->id;
+    if (static_flow) *static_flow = f;
+    return DMF_OK;
+  });
 }
 
 int dmf_server_submit(dmf_server* S, int32_t session, int32_t k, const int32_t* src, const int32_t* dst,
                       const int32_t* w, uint64_t* ticket) {
-  if (!S || !ticket || k < 0 || (k > 0 && (!src || !dst || !w))) return DMF_EINVAL;
-  auto r = std::make_unique<Request>();
-  r->src.assign(src, src + k);
-  r->dst.assign(dst, dst + k);
-  r->w.assign(w, w + k);
-  r->bytes = std::max<int64_t>(256, 3 * align256(static_cast<int64_t>(k) * 4));
-  std::lock_guard<std::mutex> lk(S->mu);
-  if (S->stopping) return DMF_EINVAL;
-  auto it = S->sessions.find(session);
-  if (it == S->sessions.end() || it->second->closing) return DMF_EINVAL;
-  if (k > it->second->max_batch || r->bytes > S->arena.size()) return DMF_EADMIT;
-  r->session = session;
-  r->ticket = S->next_ticket++;
-  r->submit_ms = S->now_ms();
-  *ticket = r->ticket;
-  S->outstanding.insert(r->ticket);
-  ++it->second->pending;
-  S->inbox.push_back(std::move(r));
-  S->cv_stage.notify_all();
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!S || !ticket || k < 0 || (k > 0 && (!src || !dst || !w))) return DMF_EINVAL;
+    auto r = std::make_unique<Request>();
+    r->src.assign(src, src + k);
+    r->dst.assign(dst, dst + k);
+    r->w.assign(w, w + k);
+    r->bytes = std::max<int64_t>(256, 3 * align256(static_cast<int64_t>(k) * 4));
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stopping) return DMF_EINVAL;
+    auto it = S->sessions.find(session);
+    if (it == S->sessions.end() || it->second->closing) return DMF_EINVAL;
+    if (k > it->second->max_batch || r->bytes > S->arena.size()) return DMF_EADMIT;
+    r->session = session;
+    r->ticket = S->next_ticket++;
+    r->submit_ms = S->now_ms();
+    *ticket = r->ticket;
+    S->outstanding.insert(r->ticket);
+    ++it->second->pending;
+    S->inbox.push_back(std::move(r));
+    S->cv_stage.notify_all();
+    return DMF_OK;
+  });
 }
 
 int dmf_server_wait(dmf_server* S, uint64_t ticket, double timeout_ms, dmf_result* out) {
-  if (!S || !out || ticket == 0) return DMF_EINVAL;
-  std::unique_lock<std::mutex> lk(S->mu);
-  if (!S->outstanding.count(ticket)) return DMF_EINVAL;  // unknown or already returned
-  auto ready = [&] { return S->results.count(ticket) != 0; };
-  if (timeout_ms < 0) {
-    S->cv_done.wait(lk, ready);
-  } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
-    return DMF_EAGAIN;
-  }
-  *out = S->results[ticket];
-  S->results.erase(ticket);
-  S->outstanding.erase(ticket);
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!S || !out || ticket == 0) return DMF_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (!S->outstanding.count(ticket)) return DMF_EINVAL;  // unknown or already returned
+    auto ready = [&] { return S->results.count(ticket) != 0; };
+    if (timeout_ms < 0) {
+      S->cv_done.wait(lk, ready);
+    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+      return DMF_EAGAIN;
+    }
+    *out = S->results[ticket];
+    S->results.erase(ticket);
+    S->outstanding.erase(ticket);
+    return DMF_OK;
+  });
 }
 
 int dmf_server_get_stats(dmf_server* S, dmf_server_stats* st) {
-  if (!S || !st) return DMF_EINVAL;
-  std::lock_guard<std::mutex> lk(S->mu);
-  std::memset(st, 0, sizeof(*st));
-  st->sessions = static_cast<int32_t>(S->sessions.size());
-  for (auto& kv : S->sessions) st->in_flight += kv.second->pending;
-  st->queued = static_cast<int32_t>(S->inbox.size());
-  st->completed = S->done_count;
-  st->hbm_budget = S->cfg.hbm_budget;
-  st->hbm_held = S->held;
-  st->staged_bytes = S->arena.used();
-  st->peak_staged_bytes = S->peak_staged;
-  st->clock_ms = S->now_ms();
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!S || !st) return DMF_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::memset(st, 0, sizeof(*st));
+    st->sessions = static_cast<int32_t>(S->sessions.size());
+    for (auto& kv : S->sessions) st->in_flight += kv.second->pending;
+    st->queued = static_cast<int32_t>(S->inbox.size());
+    st->completed = S->done_count;
+    st->hbm_budget = S->cfg.hbm_budget;
+    st->hbm_held = S->held;
+    st->staged_bytes = S->arena.used();
+    st->peak_staged_bytes = S->peak_staged;
+    st->clock_ms = S->now_ms();
+    return DMF_OK;
+  });
 }
 
 int dmf_server_close(dmf_server* S, int32_t session) {
-  if (!S) return DMF_EINVAL;
-  std::unique_ptr<Session> ss;
-  {
-    std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->sessions.find(session);
-    if (it == S->sessions.end()) return DMF_EINVAL;
-    Session* raw = it->second.get();
-    raw->closing = true;  // no new submissions; queued ones still run
-    S->cv_done.wait(lk, [&] { return raw->pending == 0; });
-    S->cv_ready.notify_all();
-  }
-  {
+  return guarded([&]() -> int {
+    if (!S) return DMF_EINVAL;
+    std::unique_ptr<Session> ss;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      auto it = S->sessions.find(session);
+      if (it == S->sessions.end()) return DMF_EINVAL;
+      Session* raw = it->second.get();
+      raw->closing = true;  // no new submissions; queued ones still run
+      S->cv_done.wait(lk, [&] { return raw->pending == 0; });
+      S->cv_ready.notify_all();
+    }
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      ss = std::move(S->sessions[session]);
+    }
+    if (ss->solver.joinable()) ss->solver.join();
+    dmf_engine_destroy(ss->eng);
     std::lock_guard<std::mutex> lk(S->mu);
-    ss = std::move(S->sessions[session]);
-  }
-  if (ss->solver.joinable()) ss->solver.join();
-  dmf_engine_destroy(ss->eng);
-  std::lock_guard<std::mutex> lk(S->mu);
-  S->held -= ss->bytes;
-  S->sessions.erase(session);
-  return DMF_OK;
+    S->held -= ss->bytes;
+    S->sessions.erase(session);
+    return DMF_OK;
+  });
 }
 
 int dmf_server_destroy(dmf_server* S) {
-  if (!S) return DMF_EINVAL;
-  std::vector<int32_t> ids;
-  {
-    std::lock_guard<std::mutex> lk(S->mu);
-    for (auto& kv : S->sessions) ids.push_back(kv.first);
-  }
-  for (int32_t id : ids) dmf_server_close(S, id);  // finishes queued batches first
-  {
-    std::lock_guard<std::mutex> lk(S->mu);
-    S->stopping = true;
-    S->cv_stage.notify_all();
-  }
-  if (S->stager.joinable()) S->stager.join();
-  (void)hipStreamDestroy(S->copy_stream);
-  (void)hipFree(S->d_arena);
-  delete S;
-  return DMF_OK;
+  return guarded([&]() -> int {
+    if (!S) return DMF_EINVAL;
+    std::vector<int32_t> ids;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (auto& kv : S->sessions) ids.push_back(kv.first);
+    }
+    for (int32_t id : ids) dmf_server_close(S, id);  // finishes queued batches first
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stopping = true;
+      S->cv_stage.notify_all();
+    }
+    if (S->stager.joinable()) S->stager.join();
+    (void)hipStreamDestroy(S->copy_stream);
+    (void)hipFree(S->d_arena);
+    delete S;
+    return DMF_OK;
+  });
 }
 
 }  // extern "C"

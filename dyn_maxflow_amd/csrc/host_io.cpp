@@ -298,6 +298,8 @@ int dmf_build_csr(int64_t k, const int32_t* u, const int32_t* v, const int32_t* 
     for (int64_t i = 0; i < k; ++i) {
       const int32_t a = u[i], b = v[i];
       if (a == b) continue;
+      // arc ids are int32: stop before one would wrap (m <= INT32_MAX below)
+      if (order.size() + 2 > static_cast<size_t>(INT32_MAX)) return DMF_EINVAL;
       bool ins = false;
       int32_t* id = arc_id.find_or_insert(pair_key(a, b), &ins);
       if (ins) {
@@ -315,7 +317,6 @@ int dmf_build_csr(int64_t k, const int32_t* u, const int32_t* v, const int32_t* 
       }
     }
     const int64_t m = static_cast<int64_t>(order.size());
-    if (m > INT32_MAX) return DMF_EINVAL;
 
     dmf_csr g{};
     g.n = static_cast<int32_t>(n);

@@ -257,6 +257,16 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_I32P)
 
 
+def _same_length(*arrays):
+    """int32 copies of parallel record arrays, refused (EINVAL) unless all are
+    one-dimensional and of one length: the library reads len(first) entries
+    of each."""
+    out = tuple(_i32(a) for a in arrays)
+    if any(a.ndim != 1 or len(a) != len(out[0]) for a in out):
+        raise DmfError(EINVAL, "record arrays differ in length or are not one-dimensional")
+    return out
+
+
 def _check(rc: int, what: str) -> None:
     if rc != OK:
         raise DmfError(rc, what)
@@ -327,7 +337,7 @@ def write_graph_bin(path: str, g: Graph) -> None:
 
 
 def build_csr(u, v, w) -> Graph:
-    u, v, w = _i32(u), _i32(v), _i32(w)
+    u, v, w = _same_length(u, v, w)
     c = _Csr()
     _check(_lib().dmf_build_csr(len(u), _ptr(u), _ptr(v), _ptr(w), C.byref(c)), "build_csr")
     return Graph._from_csr(c)
@@ -342,7 +352,7 @@ def read_updates_bin(path: str):
 
 
 def write_updates_bin(path: str, src, dst, w) -> None:
-    src, dst, w = _i32(src), _i32(dst), _i32(w)
+    src, dst, w = _same_length(src, dst, w)
     _check(_lib().dmf_write_updates_bin(path.encode(), len(src), _ptr(src), _ptr(dst), _ptr(w)),
            f"write {path}")
 
@@ -373,7 +383,7 @@ def gen_grid(rows: int, cols: int, seed: int, cap_lo=1, cap_hi=1000):
 
 
 def pick_source_sink(n: int, u, v):
-    u, v = _i32(u), _i32(v)
+    u, v = _same_length(u, v)
     s, t = C.c_int32(), C.c_int32()
     _check(_lib().dmf_pick_source_sink(n, len(u), _ptr(u), _ptr(v), C.byref(s), C.byref(t)),
            "pick_source_sink")
@@ -383,8 +393,9 @@ def pick_source_sink(n: int, u, v):
 def gen_updates(u, v, cur_w: np.ndarray, count: int, seed: int, inc_frac=0.5, cap_lo=1, cap_hi=100,
                 apply=True):
     """One batch of distinct-edge capacity updates; cur_w (int32) is updated in place if apply."""
-    u, v = _i32(u), _i32(v)
-    assert cur_w.dtype == np.int32 and cur_w.flags.c_contiguous
+    u, v = _same_length(u, v)
+    if cur_w.dtype != np.int32 or not cur_w.flags.c_contiguous or cur_w.shape != u.shape:
+        raise DmfError(EINVAL, "cur_w must be a contiguous int32 array with one entry per edge")
     k = C.c_int32()
     a, b, c = _I32P(), _I32P(), _I32P()
     _check(_lib().dmf_gen_updates(len(u), _ptr(u), _ptr(v), _ptr(cur_w), count, inc_frac, cap_lo,
@@ -452,14 +463,14 @@ class Engine:
         return f.value
 
     def apply_batch(self, src, dst, w, params: Params) -> int:
-        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        src, dst, w = _same_length(src, dst, w)
         f = C.c_int64()
         _check(_lib().dmf_engine_apply_batch(self._h, len(src), _ptr(src), _ptr(dst), _ptr(w),
                                              C.byref(params), C.byref(f)), "apply_batch")
         return f.value
 
     def apply_batch_status(self, src, dst, w, params: Params) -> int:
-        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        src, dst, w = _same_length(src, dst, w)
         f = C.c_int64()
         return _lib().dmf_engine_apply_batch(self._h, len(src), _ptr(src), _ptr(dst), _ptr(w),
                                              C.byref(params), C.byref(f))
@@ -555,14 +566,14 @@ class Server:
         return sid.value, f.value
 
     def submit(self, session: int, src, dst, w) -> int:
-        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        src, dst, w = _same_length(src, dst, w)
         tk = C.c_uint64()
         _check(_lib().dmf_server_submit(self._h, session, len(src), _ptr(src), _ptr(dst), _ptr(w),
                                         C.byref(tk)), "server_submit")
         return tk.value
 
     def submit_status(self, session: int, src, dst, w) -> int:
-        src, dst, w = _i32(src), _i32(dst), _i32(w)
+        src, dst, w = _same_length(src, dst, w)
         tk = C.c_uint64()
         return _lib().dmf_server_submit(self._h, session, len(src), _ptr(src), _ptr(dst), _ptr(w), C.byref(tk))
 
@@ -603,12 +614,13 @@ class BatchEngine(Engine):
     (dmf_batch_create); every solve covers all of them at once."""
 
     def __init__(self, graphs, sources, sinks):
+        self._h = C.c_void_p()
         self.graphs = list(graphs)
         k = len(self.graphs)
+        s, t = _same_length(sources, sinks)
+        if len(s) != k:
+            raise DmfError(EINVAL, f"batch_create takes one source and one sink per graph ({k})")
         self._csrs = (_Csr * k)(*[g._csr() for g in self.graphs])
-        s = _i32(sources)
-        t = _i32(sinks)
-        self._h = C.c_void_p()
         _check(_lib().dmf_batch_create(k, self._csrs, _ptr(s), _ptr(t), C.byref(self._h)), "batch_create")
         self.k = k
         self.offsets = np.concatenate([[0], np.cumsum([g.n for g in self.graphs])]).astype(np.int64)
@@ -624,10 +636,13 @@ class BatchEngine(Engine):
 
     def apply_all(self, updates, params: Params) -> np.ndarray:
         """updates: one (src, dst, w) per instance, instance-local ids."""
-        counts = _i32([len(u[0]) for u in updates])
-        src = _i32(np.concatenate([u[0] for u in updates])) if len(updates) else _i32([])
-        dst = _i32(np.concatenate([u[1] for u in updates])) if len(updates) else _i32([])
-        w = _i32(np.concatenate([u[2] for u in updates])) if len(updates) else _i32([])
+        if len(updates) != self.k:
+            raise DmfError(EINVAL, f"apply_all takes one (src, dst, w) per instance ({self.k})")
+        recs = [_same_length(*u) for u in updates]
+        counts = _i32([len(r[0]) for r in recs])
+        src = _i32(np.concatenate([r[0] for r in recs]))
+        dst = _i32(np.concatenate([r[1] for r in recs]))
+        w = _i32(np.concatenate([r[2] for r in recs]))
         f = np.zeros(self.k, np.int64)
         _check(_lib().dmf_batch_apply(self._h, _ptr(counts), _ptr(src), _ptr(dst), _ptr(w),
                                       C.byref(params), f.ctypes.data_as(_I64P)), "batch_apply")
@@ -648,7 +663,7 @@ class BatchEngine(Engine):
 
 
 def recalculate_max_flow(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="data"):
-    src, dst, w = _i32(src), _i32(dst), _i32(w)
+    src, dst, w = _same_length(src, dst, w)
     c = g._csr()
     f = C.c_int64()
     ms_s, ms_b = C.c_double(), C.c_double()
@@ -660,7 +675,7 @@ def recalculate_max_flow(g: Graph, src, dst, w, s: int, t: int, kc: int, variant
 
 
 def static_recalculate(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="data"):
-    src, dst, w = _i32(src), _i32(dst), _i32(w)
+    src, dst, w = _same_length(src, dst, w)
     c = g._csr()
     f = C.c_int64()
     ms = C.c_double()

@@ -79,3 +79,25 @@ def test_server_refuses_without_device():
     with pytest.raises(dmf.DmfError) as ex:
         dmf.Server(dmf.server_config())
     assert ex.value.status == dmf.EDEVICE
+
+
+def test_mirror_refuses_mismatched_record_arrays():
+    """The library reads len(src) entries of every record array; the mirror
+    refuses arrays that disagree before anything reaches it."""
+    with pytest.raises(dmf.DmfError) as e:
+        dmf.build_csr([0, 1, 2], [1, 2], [5, 5, 5])
+    assert e.value.status == dmf.EINVAL
+    with pytest.raises(dmf.DmfError) as e:
+        dmf.pick_source_sink(3, [0, 1], [1])
+    assert e.value.status == dmf.EINVAL
+    u, v, w = dmf.gen_rmat(6, 4, 1)
+    with pytest.raises(dmf.DmfError) as e:
+        dmf.gen_updates(u, v, w[:-1].copy(), 5, 1)
+    assert e.value.status == dmf.EINVAL
+    g = dmf.build_csr(u, v, w)
+    with pytest.raises(dmf.DmfError) as e:
+        dmf.recalculate_max_flow(g, [0, 1], [1], [3, 3], 0, 1, kc=1)
+    assert e.value.status == dmf.EINVAL
+    with pytest.raises(dmf.DmfError) as e:
+        dmf.BatchEngine([g, g], [0, 0], [1])
+    assert e.value.status == dmf.EINVAL

@@ -162,6 +162,10 @@ def summarize(recs):
         "rounds_mean": float(np.mean([r["rounds"] for r in recs])),
         "launches_per_batch": float(np.mean([r["launches"] for r in recs])),
         "global_relabels_mean": float(np.mean([r["global_relabels"] for r in recs])),
+        # atomic operations issued by the kernels (claims, excess transfers,
+        # queue reservations), counted on the device
+        "atomics_per_batch": float(np.mean([r["atomics"] for r in recs])),
+        "atomic_gops": sum(r["atomics"] for r in recs) / tsec / 1e9 if tsec > 0 else 0.0,
     }
 
 

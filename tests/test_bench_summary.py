@@ -11,8 +11,11 @@ import bench  # noqa: E402
 
 def _record(ms, arcs, atomics):
     r = dmf.Counters().as_dict()  # exactly the fields counters() returns
-    r.update(device_ms=ms, arc_visits=arcs, vertex_visits=arcs // 4, updates=100, rounds=3, launches=40,
-             global_relabels=2, atomics=atomics, wall_ms=ms + 0.1, batch=0, flow=1)
+    counted = dict(device_ms=ms, arc_visits=arcs, vertex_visits=arcs // 4, updates=100, rounds=3, launches=40,
+                   global_relabels=2, atomics=atomics)
+    assert set(counted) <= set(r), set(counted) - set(r)
+    r.update(counted)
+    r.update(wall_ms=ms + 0.1, batch=0, flow=1)  # added by bench.run_gpu
     return r
 
 

@@ -203,6 +203,7 @@ def _lib():
                                                     C.c_void_p, C.POINTER(Params), _I64P]),
         "dmf_engine_flow": (C.c_int, [C.c_void_p, _I64P]),
         "dmf_engine_min_cut": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+        "dmf_engine_complete_flow": (C.c_int, [C.c_void_p, _I64P]),
         "dmf_engine_read_state": (C.c_int, [C.c_void_p, _I32P, _I32P, _I64P, _I32P]),
         "dmf_engine_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters)]),
         "dmf_engine_device_bytes": (C.c_int64, [C.c_void_p]),
@@ -492,6 +493,12 @@ class Engine:
         _check(_lib().dmf_engine_min_cut(self._h, side.ctypes.data_as(C.POINTER(C.c_uint8))), "min_cut")
         return side
 
+    def complete_flow(self) -> int:
+        """Maximum preflow -> maximum flow of the same value (dmf_engine_complete_flow)."""
+        f = C.c_int64()
+        _check(_lib().dmf_engine_complete_flow(self._h, C.byref(f)), "complete_flow")
+        return f.value
+
     def state(self):
         res = np.zeros(self.g.m, np.int32)
         cap = np.zeros(self.g.m, np.int32)
@@ -683,3 +690,4 @@ def static_recalculate(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="
     _check(_lib().dmf_static_recalculate(C.byref(c), len(src), _ptr(src), _ptr(dst), _ptr(w), s, t, kc,
                                          var, C.byref(f), C.byref(ms)), "static_recalculate")
     return f.value, ms.value
+

@@ -103,11 +103,11 @@ struct dmf_engine {
   // graph-replayed BFS (mode 2): segments of 2^i level pairs per BFS mode
   static constexpr int kSegSizes = 7;
   Dev* d_dev = nullptr;
-  hipGraph_t seg_graph[3][kSegSizes] = {};
-  hipGraphExec_t seg_exec[3][kSegSizes] = {};
+  hipGraph_t seg_graph[kBfsModes][kSegSizes] = {};
+  hipGraphExec_t seg_exec[kBfsModes][kSegSizes] = {};
   bool use_graph = false;
   bool graph_failed = false;
-  int seg_levels[3] = {0, 0, 0};
+  int seg_levels[kBfsModes] = {};
   // per-launch BFS level timing (dmf_engine_set_profiling)
   bool profile = false;
   hipEvent_t pev0 = nullptr, pev1 = nullptr;
@@ -260,7 +260,7 @@ int capture_segment(dmf_engine* E, int mode, int idx) {
 }
 
 void destroy_segments(dmf_engine* E) {
-  for (int a = 0; a < 3; ++a)
+  for (int a = 0; a < kBfsModes; ++a)
     for (int b = 0; b < dmf_engine::kSegSizes; ++b) {
       if (E->seg_exec[a][b]) (void)hipGraphExecDestroy(E->seg_exec[a][b]);
       if (E->seg_graph[a][b]) (void)hipGraphDestroy(E->seg_graph[a][b]);
@@ -675,6 +675,30 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
     std::fflush(E->trace);
   }
   return rc;
+}
+
+// One phase of the preflow-to-flow completion.  deficits: every non-terminal
+// deficit pulls flow back along the forward residual graph from the sinks
+// (BFS_CT labels, never from a source); otherwise every non-terminal excess is
+// pushed back along the residual graph to the sources (BFS_CS labels, never
+// into a sink).  Labels are recomputed before every round, so the rounds need
+// no label repair; the phase ends when no labelled vertex is imbalanced.
+int complete_phase(dmf_engine* E, bool deficits, int kc, int max_rounds) {
+  Dev& D = E->D;
+  const int mode = deficits ? BFS_CT : BFS_CS;
+  for (int round = 0;; ++round) {
+    int visited = 0, rc;
+    if ((rc = relabel(E, mode, 0, &visited))) return rc;
+    if ((rc = reset_lists(E))) return rc;
+    LAUNCH(k_build_imbalance_wl, blocks_for(visited), D, visited, deficits ? 1 : 0);
+    if ((rc = sync_ctrl(E))) return rc;
+    const int32_t cnt = deficits ? E->h_ctrl->wl2_cnt : E->h_ctrl->wl_cnt;
+    const unsigned long long wpk = E->h_ctrl->wq[deficits ? kQPull : kQActive];
+    if (cnt == 0 && wpk == 0) return DMF_OK;
+    if (round >= max_rounds) return DMF_ENOCONV;
+    if (deficits) LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 2, cnt, kc, PM_NO_SRC, wpk);
+    else LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, kc, PM_NO_SNK, wpk);
+  }
 }
 
 int ensure_update_capacity(dmf_engine* E, int32_t k) {
@@ -1307,18 +1331,21 @@ int dmf_engine_min_cut(dmf_engine* E, uint8_t* side) {
   return guarded([&]() -> int {
     if (!E || !side) return DMF_EINVAL;
     Dev& D = E->D;
-    int32_t* mark = D.wl2;  // scratch between solves
+    // scratch between solves: the worklists, not order[] (a lazily started
+    // batch begins from the labelled set order[0, visited) of the last solve)
+    int32_t* mark = D.wl2;
+    int32_t* queue = D.wl;
     int rc = DMF_OK;
     do {
       if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) break;
-      LAUNCH(k_cut_init, blocks_for(E->n), D, mark);
+      LAUNCH(k_cut_init, blocks_for(E->n), D, mark, queue);
       int lo = 0, lvl = 0;
       for (;;) {
         int32_t cnt = 0;
         if ((rc = read_int(E, &D.ctrl->qcnt[lvl % 3], &cnt))) break;
         if (cnt == 0) break;
         if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) break;
-        LAUNCH(k_cut_level, blocks_for(cnt), D, mark, lo, cnt, (lvl + 1) % 3);
+        LAUNCH(k_cut_level, blocks_for(cnt), D, mark, queue, lo, cnt, (lvl + 1) % 3);
         lo += cnt;
         ++lvl;
       }
@@ -1336,6 +1363,38 @@ int dmf_engine_min_cut(dmf_engine* E, uint8_t* side) {
       }
     } while (false);
     return rc;
+  });
+}
+
+int dmf_engine_complete_flow(dmf_engine* E, int64_t* flow) {
+  return guarded([&]() -> int {
+    if (!E) return DMF_EINVAL;
+    Dev& D = E->D;
+    // the solve's counters stay those of the last solve
+    const dmf_counters saved = E->last;
+    const uint64_t launches = E->launches;
+    E->labels_fresh = false;  // heights become distances to the terminals
+    const int kc = std::max(1, E->m / std::max(1, E->n));
+    const int cap = static_cast<int>(std::min<int64_t>(4LL * E->n + 1000, INT32_MAX));
+    int rc = complete_phase(E, true, kc, cap);
+    if (!rc) rc = complete_phase(E, false, kc, cap);
+    if (!rc) rc = reset_lists(E);
+    if (!rc) rc = zero_dev(E, &D.ctrl->part_cnt, sizeof(int32_t));
+    if (!rc) {
+      LAUNCH(k_count_imbalance, blocks_for(E->n), D);
+      rc = sync_ctrl(E);
+    }
+    int32_t left = 0;
+    if (!rc) {
+      left = E->h_ctrl->part_cnt;
+      rc = zero_dev(E, &D.ctrl->part_cnt, sizeof(int32_t));
+    }
+    E->last = saved;
+    E->launches = launches;
+    if (rc) return rc;
+    if (left) return DMF_ENOCONV;
+    if (!flow) return DMF_OK;
+    return dmf_engine_flow(E, flow);
   });
 }
 

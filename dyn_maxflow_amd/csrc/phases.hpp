@@ -572,8 +572,14 @@ __device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
 }
 
 // ----------------------------------------------------------- global relabel
-// Root predicates of the three global relabels.
-enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2 };
+// Root predicates of the three global relabels, plus the two labellings of
+// the preflow-to-flow completion: BFS_CT labels the distance from the sinks
+// (and from stuck positive excesses) in the forward residual graph, along
+// which a deficit pulls flow back; BFS_CS labels the distance to the sources
+// in the residual graph, along which stuck excess is returned.  Neither passes
+// through a terminal.
+enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2, BFS_CT = 3, BFS_CS = 4 };
+constexpr int kBfsModes = 5;
 
 __device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1] - D.row[v]; }
 
@@ -590,6 +596,10 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
       const long long ex = D.excess[v];
       if (mode == BFS_SINK) {
         root = is_snk(D, v) || (!is_src(D, v) && ex < 0);
+      } else if (mode == BFS_CT) {
+        root = is_snk(D, v) || (!is_term(D, v) && ex > 0);
+      } else if (mode == BFS_CS) {
+        root = is_src(D, v);
       } else {
         const bool inB = D.side[v] != 0;
         root = is_term(D, v) || (inB && ex < 0) || (!inB && ex > 0);
@@ -659,11 +669,12 @@ __device__ __forceinline__ int bfs_resid(const Dev& D, int mode, bool vB, int e)
     const int32_t back = D.rres[e], fwd = D.res[e];
     return vB ? back : fwd;
   }
+  if (mode == BFS_CT) return D.res[e];  // mode is uniform over the launch
   return D.rres[e];
 }
 __device__ __forceinline__ bool bfs_head_ok(const Dev& D, int mode, int w) {
   if (mode == BFS_SINK) return true;
-  if (mode == BFS_PP) return !is_term(D, w);
+  if (mode == BFS_PP || mode == BFS_CT || mode == BFS_CS) return !is_term(D, w);
   return !D.side[w] && D.pfinal[w];
 }
 
@@ -944,6 +955,44 @@ __device__ void ph_build_pp_wl(const Dev& D, const Range& r, int cnt, Tally& t) 
   }
 }
 
+// Worklist of the preflow-to-flow completion over the labelled vertices
+// order[0,cnt): with deficits set, non-terminal vertices with excess < 0 (to
+// wl2 and the pull queue), else those with excess > 0 (to wl and the active
+// queue); vertices above lane_max go to the chunked queue.
+__device__ void ph_build_imbalance_wl(const Dev& D, const Range& r, int cnt, bool deficits, Tally& t) {
+  int32_t* list = deficits ? D.wl2 : D.wl;
+  int32_t* lcnt = deficits ? &D.ctrl->wl2_cnt : &D.ctrl->wl_cnt;
+  const int q = deficits ? kQPull : kQActive;
+  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+    const int i = base + lane_id();
+    bool act = false, wide = false;
+    int v = -1, d = 0;
+    if (i < cnt) {
+      v = D.order[i];
+      const long long ex = D.excess[v];
+      act = !is_term(D, v) && (deficits ? ex < 0 : ex > 0) && D.height[v] < D.n;
+      t.vert += 1;
+      if (act) {
+        d = degree(D, v);
+        wide = d > D.lane_max;
+        act = !wide;
+      }
+    }
+    wave_wq_append(D, q, wide, v, d);
+    int rank;
+    const int off = wave_reserve(lcnt, act, &rank);
+    if (act) list[off + rank] = v;
+  }
+}
+
+// Non-terminal vertices whose excess is not zero (after a completion: none).
+__device__ void ph_count_imbalance(const Dev& D, const Range& r) {
+  int bad = 0;
+  for (int v = r.gid; v < D.n; v += r.gsz) bad += (!is_term(D, v) && D.excess[v] != 0) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o);
+  if (lane_id() == 0 && bad) atomicAdd(&D.ctrl->part_cnt, bad);
+}
+
 // create_final_wl (code/dyn_pp.h:619-642) over the part list.
 __device__ void ph_build_final_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
   for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
@@ -968,12 +1017,16 @@ __device__ void ph_build_final_wl(const Dev& D, const Range& r, int cnt, Tally& 
 }
 
 // ---------------------------------------------------------- push / pull
-// Neighbour filters: all, B only (pp push side), A only (pp pull side).
-enum PushMode { PM_ALL = 0, PM_ONLY_B = 1, PM_ONLY_A = 2 };
+// Neighbour filters: all, B only (pp push side), A only (pp pull side), and
+// the completion's: never pull from a source, never push into a sink (either
+// would change the flow value).
+enum PushMode { PM_ALL = 0, PM_ONLY_B = 1, PM_ONLY_A = 2, PM_NO_SRC = 3, PM_NO_SNK = 4 };
 
 __device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
   if (pmode == PM_ONLY_B) return D.side[w] != 0;
   if (pmode == PM_ONLY_A) return D.side[w] == 0;
+  if (pmode == PM_NO_SRC) return !is_src(D, w);
+  if (pmode == PM_NO_SNK) return !is_snk(D, w);
   return true;
 }
 
@@ -1498,7 +1551,7 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
   for (int i = r.gid; i < D.k; i += r.gsz) {
     const int a = D.usrc[i], b = D.udst[i];
     int arc = -2;
-    if (a < 0 || a >= D.n || b < 0 || b >= D.n) {
+    if (a < 0 || a >= D.n || b < 0 || b >= D.n || D.uw[i] < 0) {  // capacities are >= 0
       ++invalid;
     } else if (a != b) {  // self-loops skipped (code/update.hpp:82, code/dyn_data.h:194)
       arc = D.vmap ? lookup_arc(D, D.vmap[a], D.vmap[b]) : lookup_arc(D, a, b);
@@ -1573,8 +1626,10 @@ __device__ void ph_upd_release(const Dev& D, const Range& r) {
 }
 
 // Reachability from s and from positive-excess vertices over res > 0 arcs
-// (S_min), one BFS level.  mark is a caller scratch array of n ints.
-__device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark) {
+// (S_min), one BFS level.  mark and queue are caller scratch arrays of n ints
+// (the queue holds each reached vertex once, frontiers as contiguous slices);
+// neither is order[], whose labelled set a lazily started batch reuses.
+__device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark, int32_t* queue) {
   for (int base = r.gid - lane_id(); base < D.n; base += r.gsz) {
     const int v = base + lane_id();
     bool root = false;
@@ -1584,18 +1639,18 @@ __device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark) {
     }
     int rank;
     const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
-    if (root) D.order[off + rank] = v;
+    if (root) queue[off + rank] = v;
   }
 }
 
-__device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int lo, int cnt,
+__device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int32_t* queue, int lo, int cnt,
                              int32_t* next_cnt) {
-  int32_t* out = D.order + lo + cnt;
+  int32_t* out = queue + lo + cnt;
   for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
     const int i = base + lane_id();
     int b = 0, deg = 0;
     if (i < cnt) {
-      const int v = D.order[lo + i];
+      const int v = queue[lo + i];
       b = D.row[v];
       deg = D.row[v + 1] - b;
     }
@@ -1748,7 +1803,7 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
   } else if (which == 1) {
     ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t);
   } else {
-    ph_push<true>(D, r, D.wl2, cnt, kc, PM_ONLY_A, nullptr, wlist(D, kQPull, wpk), t);
+    ph_push<true>(D, r, D.wl2, cnt, kc, pmode, nullptr, wlist(D, kQPull, wpk), t);
   }
   t.flush(D.ctrs);
 }
@@ -1784,10 +1839,19 @@ __global__ void __launch_bounds__(kBlock) k_upd_clamp(Dev D) {
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_upd_release(Dev D) { ph_upd_release(D, make_range()); }
-__global__ void __launch_bounds__(kBlock) k_cut_init(Dev D, int32_t* mark) { ph_cut_init(D, make_range(), mark); }
-__global__ void __launch_bounds__(kBlock) k_cut_level(Dev D, int32_t* mark, int lo, int cnt, int slot) {
-  ph_cut_level(D, make_range(), mark, lo, cnt, &D.ctrl->qcnt[slot]);
+__global__ void __launch_bounds__(kBlock) k_cut_init(Dev D, int32_t* mark, int32_t* queue) {
+  ph_cut_init(D, make_range(), mark, queue);
 }
+__global__ void __launch_bounds__(kBlock) k_cut_level(Dev D, int32_t* mark, int32_t* queue, int lo, int cnt,
+                                                      int slot) {
+  ph_cut_level(D, make_range(), mark, queue, lo, cnt, &D.ctrl->qcnt[slot]);
+}
+__global__ void __launch_bounds__(kBlock) k_build_imbalance_wl(Dev D, int cnt, int deficits) {
+  Tally t;
+  ph_build_imbalance_wl(D, make_range(), cnt, deficits != 0, t);
+  t.flush(D.ctrs);
+}
+__global__ void __launch_bounds__(kBlock) k_count_imbalance(Dev D) { ph_count_imbalance(D, make_range()); }
 
 // Builds the (tail, head) -> arc hash index, one thread per arc; for a repeated
 // (tail, head) pair the highest arc index wins, as the reference's last-match scan.

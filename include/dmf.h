@@ -214,8 +214,9 @@ int dmf_engine_static_solve(dmf_engine* e, const dmf_params* p, int64_t* flow);
  * OnAdd_kernel + Incremental (code/dyn_data.h:186-206,
  * code/dyn_data_bench.cu:98-170; topo/pp counterparts).  Host arrays; the copy
  * to HBM is outside the device-time span.  Duplicate arcs in one batch: the
- * last record wins.  A missing arc fails the whole batch with DMF_ENOARC and
- * leaves the state untouched; self-loops are skipped. */
+ * last record wins.  A missing arc fails the whole batch with DMF_ENOARC, and
+ * an out-of-range vertex id or a negative capacity with DMF_EINVAL; either
+ * leaves the state untouched.  Self-loops are skipped. */
 int dmf_engine_apply_batch(dmf_engine* e, int32_t k, const int32_t* src,
                            const int32_t* dst, const int32_t* w,
                            const dmf_params* p, int64_t* flow);
@@ -230,9 +231,27 @@ int dmf_engine_apply_batch_device(dmf_engine* e, int32_t k, const int32_t* d_src
 int dmf_engine_flow(dmf_engine* e, int64_t* flow);
 
 /* Source side of the minimum cut S_min (vertices reachable from s in the
- * residual graph of a maximum flow), side[v] = 1 for v in S_min.  No reference
- * counterpart (the reference never reports a cut set). */
+ * residual graph of a maximum flow), side[v] = 1 for v in S_min.  On a
+ * preflow the search also starts from every non-sink vertex with positive
+ * excess (that excess flows back to s once the flow is completed, so the set
+ * is the same); after dmf_engine_complete_flow it is plain reachability from
+ * the sources.  No reference counterpart (the reference never reports a cut
+ * set). */
 int dmf_engine_min_cut(dmf_engine* e, uint8_t* side);
+
+/* Turns the maximum preflow of the last solve into a maximum flow of the same
+ * value, on the device: every non-terminal deficit is cancelled by pulling
+ * flow back from the sinks, and every non-terminal excess stranded on the
+ * source side is returned to the sources.  Afterwards excess[v] == 0 for every
+ * v that is neither a source nor a sink, excess[t] is the flow value and the
+ * per-arc flows cap - res satisfy capacity and conservation.  *flow (may be
+ * NULL) receives the value, equal to dmf_engine_flow before the call.  Runs
+ * outside any batch; later batches continue from the completed state.
+ * DMF_ENOCONV if an imbalance cannot be routed (only possible when the last
+ * solve did not converge).  Supersedes the deficit fold of
+ * recalculate_max_flow_kernel_39 (code/dyn_data.h:208-215), which leaves the
+ * state a preflow. */
+int dmf_engine_complete_flow(dmf_engine* e, int64_t* flow);
 
 /* Copy device state to host for checking: res/cap per arc (m), excess per
  * vertex (n), height per vertex (n).  Any pointer may be NULL. */

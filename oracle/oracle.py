@@ -170,18 +170,46 @@ def max_threads() -> int:
 
 def check_state(g, res, cap, excess, s=None):
     """Residual-state invariants: 0 <= res, pair consistency res[e]+res[rev]=cap[e]+cap[rev],
-    excess[v] = net inflow = sum over out-arcs of (res - cap).  Returns a list of violations."""
+    excess[v] = net inflow = sum over out-arcs of (res - cap) (exact int64, as differences of
+    a running sum over the CSR).  Returns a list of violations."""
     bad = []
-    res = res.astype(np.int64)
-    cap = cap.astype(np.int64)
-    rev = g.rev.astype(np.int64)
     if (res < 0).any():
         bad.append(f"negative residual on {(res < 0).sum()} arcs")
-    if not np.array_equal(res + res[rev], cap + cap[rev]):
+    # res[e] + res[rev] == cap[e] + cap[rev]  <=>  d[e] == -d[rev] with d = res - cap
+    d = res.astype(np.int64) - cap.astype(np.int64)
+    if not np.array_equal(d, -d[g.rev]):
         bad.append("arc pair residuals do not sum to the pair capacity")
-    tails = g.tails()
-    net = np.zeros(g.n, np.int64)
-    np.add.at(net, tails, res - cap)
-    if not np.array_equal(net, excess.astype(np.int64)):
+    csum = np.concatenate([[0], np.cumsum(d)])
+    row = g.row.astype(np.int64)
+    if not np.array_equal(csum[row[1:]] - csum[row[:-1]], excess.astype(np.int64)):
         bad.append("excess differs from net inflow")
     return bad
+
+
+def check_flow(g, res, cap, excess, sources, sinks, flow):
+    """A flow, not a preflow: check_state, plus conservation (excess 0 at every
+    vertex that is neither a source nor a sink) and excess of the sinks summing
+    to the flow value, of the sources to minus it.  Returns a list of violations."""
+    bad = check_state(g, res, cap, excess)
+    ex = excess.astype(np.int64)
+    term = np.zeros(g.n, bool)
+    term[np.asarray(sources)] = True
+    term[np.asarray(sinks)] = True
+    off = np.flatnonzero(~term & (ex != 0))
+    if len(off):
+        bad.append(f"conservation fails at {len(off)} vertices (first {off[:5].tolist()})")
+    if int(ex[np.asarray(sinks)].sum()) != flow:
+        bad.append(f"sink excess {int(ex[np.asarray(sinks)].sum())} != flow {flow}")
+    if int(ex[np.asarray(sources)].sum()) != -flow:
+        bad.append(f"source excess {int(ex[np.asarray(sources)].sum())} != -flow {-flow}")
+    return bad
+
+
+def cut_capacity(g, cap, side):
+    """Capacity of the cut (S, V \\ S) with S = {v : side[v] != 0}: sum of cap over
+    arcs leaving S.  Equal to the value of a feasible flow, it certifies that
+    flow maximum and the cut minimum (max-flow = min-cut)."""
+    inS = np.asarray(side).astype(bool)
+    tails = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.row))
+    leave = inS[tails] & ~inS[g.col]
+    return int(cap.astype(np.int64)[leave].sum())

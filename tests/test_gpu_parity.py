@@ -139,6 +139,12 @@ def test_batch_edge_cases(gpu, mode):
         rc = eng.apply_batch_status([g.n + 5], [0], [1], p)
         assert rc == dmf.EINVAL
         assert eng.flow() == f0
+        # a negative capacity is refused like a malformed CSR, state untouched
+        rc = eng.apply_batch_status([int(u[0]), int(u[1])], [int(v[0]), int(v[1])], [5, -3], p)
+        assert rc == dmf.EINVAL
+        assert eng.flow() == f0
+        _, capn, _, _ = eng.state()
+        assert np.array_equal(capn, g.cap)
         # out-of-range parameters are refused before any device work
         for field, val in (("mode", 7), ("mode", -1), ("variant", 9), ("scheduler", 4)):
             bad = dmf.default_params(g, mode=mode)
@@ -442,3 +448,60 @@ def test_vertex_scatter_is_invisible(gpu, mode, monkeypatch):
     assert seqs["0"][0] == seqs["1"][0]
     for a, b in zip(seqs["0"][1], seqs["1"][1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_completion_after_every_batch(gpu, variant, mode):
+    """The maximum preflow of every solve is completed into a maximum flow on
+    the device: same value, conservation at every non-terminal vertex, same
+    S_min; the next batch continues from the completed state and is exact."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 3)
+    p = dmf.default_params(g, variant=variant, mode=mode)
+    say(f"[complete] variant={variant} mode={mode}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(p)
+        applied = []
+        cur = w.copy()
+        for b in range(5):
+            cap = caps_after(g, applied)
+            check_engine(eng, g, cap, s, t, f, f"before completion {b}")
+            side = eng.min_cut()
+            assert eng.complete_flow() == f
+            res, capd, ex, _ = eng.state()
+            bad = oracle.check_flow(g, res, capd, ex, [s], [t], f)
+            assert not bad, f"completed {b}: {bad}"
+            assert np.array_equal(eng.min_cut(), side)
+            assert eng.complete_flow() == f  # idempotent
+            applied.append(dmf.gen_updates(u, v, cur, len(u) * (b + 2) // 100, 300 + b))
+            f = eng.apply_batch(*applied[-1], p)
+        check_engine(eng, g, caps_after(g, applied), s, t, f, "last batch")
+
+
+def test_lazy_and_learned_batches_without_min_cut(gpu):
+    """Lazily started and learned-scheduler batches, with no min_cut (or any
+    other read) between them, end exact, as do the same batches with S_min
+    read between them.  (Round and relabel counts are not compared: the
+    lock-free pushes make them vary from run to run; the flows do not.)"""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 11)
+    cur = w.copy()
+    bl = [dmf.gen_updates(u, v, cur, len(u) * pc // 100, 700 + i) for i, pc in enumerate((2, 5, 1, 7, 3))]
+    runs = {}
+    for sched in ("lazy", "learned"):
+        for between in (False, True):
+            p = (dmf.default_params(g, mode=1, lazy_start=1) if sched == "lazy"
+                 else dmf.default_params(g, mode=1, scheduler=dmf.SCHED_LEARNED))
+            say(f"[no-min-cut] {sched} min_cut between batches: {between}")
+            ctr = []
+            with dmf.Engine(g, s, t) as eng:
+                eng.static_solve(dmf.default_params(g, variant="pp", mode=1))
+                for b in bl:
+                    f = eng.apply_batch(*b, p)
+                    ctr.append(f)
+                    if between:
+                        eng.min_cut()
+                check_engine(eng, g, caps_after(g, bl), s, t, f, f"{sched} last batch")
+            runs[(sched, between)] = ctr
+        assert runs[(sched, False)] == runs[(sched, True)], sched

@@ -221,3 +221,71 @@ def test_edge_cases_of_the_restatement():
     cap = pr.state()[2]
     e = [i for i in range(g.row[a], g.row[a + 1]) if g.col[i] == b][-1]
     assert cap[e] == 77
+
+
+def _path_flow_state(g, flows):
+    """Residuals of a flow given per (tail, head) input edge: res[e] = cap[e] - f(e)
+    on the edge's arc and cap[rev] + f(e) on its reverse."""
+    res = g.cap.astype(np.int64).copy()
+    tails = g.tails()
+    for (a, b), f in flows.items():
+        e = int(np.flatnonzero((tails == a) & (g.col == b))[0])
+        res[e] -= f
+        res[g.rev[e]] += f
+    ex = np.zeros(g.n, np.int64)
+    for (a, b), f in flows.items():
+        ex[b] += f
+        ex[a] -= f
+    return res.astype(np.int32), ex
+
+
+def test_check_flow_and_cut_capacity_pinned():
+    """check_flow accepts a hand-built maximum flow and catches a planted
+    conservation error; cut_capacity equals hand-computed cut sums and, for
+    S_min, Dinic's value (max-flow = min-cut) on random graphs."""
+    # s=0 -> 1 -> 3=t and 0 -> 2 -> 3, capacities 5/3 and 4/6: max flow 3 + 4 = 7
+    g = dmf.build_csr([0, 1, 0, 2], [1, 3, 2, 3], [5, 3, 4, 6])
+    res, ex = _path_flow_state(g, {(0, 1): 3, (1, 3): 3, (0, 2): 4, (2, 3): 4})
+    assert oracle.check_flow(g, res, g.cap, ex, [0], [3], 7) == []
+    assert oracle.dinic(g, 0, 3)[0] == 7
+    # planted: one unit more into vertex 1 than leaves it
+    res2, ex2 = _path_flow_state(g, {(0, 1): 4, (1, 3): 3, (0, 2): 4, (2, 3): 4})
+    bad = oracle.check_flow(g, res2, g.cap, ex2, [0], [3], 7)
+    assert any("conservation" in b for b in bad) and any("source excess" in b for b in bad)
+    # planted: the right state reported with a wrong value
+    assert any("sink excess" in b for b in oracle.check_flow(g, res, g.cap, ex, [0], [3], 8))
+    # cut sums by hand: {0} -> 5 + 4, {0,1} -> 3 + 4, {0,1,2} -> 3 + 6
+    for S, c in (({0}, 9), ({0, 1}, 7), ({0, 1, 2}, 9)):
+        side = np.zeros(g.n, np.uint8)
+        side[list(S)] = 1
+        assert oracle.cut_capacity(g, g.cap, side) == c == cut_capacity(g, side)
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        n = int(rng.integers(5, 50))
+        gr, _ = small_random_graph(rng, n, int(rng.integers(n, 5 * n)))
+        f, side = oracle.dinic(gr, 0, n - 1)
+        assert oracle.cut_capacity(gr, gr.cap, side) == f
+
+
+def test_small_config_on_the_cpu_path():
+    """The small plain configuration run on the CPU push-relabel path: RMAT-14,
+    EF16, seed 1; static solve, then 10 batches of 1% of the input edges, flow
+    value and S_min equal to Dinic after every batch (data variant, kc = m/n)."""
+    g, s, t, (u, v, w) = rmat_case(14, 16, 1)
+    kc = max(1, g.m // g.n)
+    pr = oracle.PushRelabel(g, s, t, threads=1)
+    try:
+        f = pr.static("data", kc)
+        fd, side = oracle.dinic(g, s, t)
+        assert f == fd and np.array_equal(pr.min_cut(), side)
+        cur = w.copy()
+        applied = []
+        for b in range(10):
+            src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 100, 100 + b)
+            applied.append((src, dst, nw))
+            f = pr.apply(src, dst, nw, "data", kc)
+            fd, side = oracle.dinic(g, s, t, cap=caps_after(g, applied))
+            assert f == fd, (b, f, fd)
+            assert np.array_equal(pr.min_cut(), side), b
+    finally:
+        pr.close()

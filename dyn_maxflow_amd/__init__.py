@@ -33,6 +33,7 @@ ENOCONV = -6
 EFORMAT = -7
 EADMIT = -8
 EAGAIN = -9
+ESIZE = -10
 
 VARIANT_DATA = 0
 VARIANT_TOPO = 1
@@ -213,6 +214,9 @@ def _lib():
         "dmf_static_recalculate": (C.c_int, [C.POINTER(_Csr), C.c_int32, _I32P, _I32P, _I32P,
                                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I64P,
                                              C.POINTER(C.c_double)]),
+        "dmf_static_recalculate_noalloc": (C.c_int, [C.POINTER(_Csr), C.c_int32, _I32P, _I32P, _I32P,
+                                                     C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                                     _I64P]),
         "dmf_device_count": (C.c_int, []),
         "dmf_engine_autotune": (C.c_int, [C.c_void_p, C.POINTER(Params), _I32P, _I32P]),
         "dmf_batch_create": (C.c_int, [C.c_int32, C.POINTER(_Csr), _I32P, _I32P, C.POINTER(C.c_void_p)]),
@@ -691,3 +695,21 @@ def static_recalculate(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="
                                          var, C.byref(f), C.byref(ms)), "static_recalculate")
     return f.value, ms.value
 
+
+def static_recalculate_noalloc_status(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="data",
+                                      stream_ptr: int = 0):
+    """dmf_static_recalculate_noalloc: (status, flow).  The library allocates
+    nothing for it (fixed workspace; DMF_ESIZE beyond its limits)."""
+    src, dst, w = _same_length(src, dst, w)
+    c = g._csr()
+    f = C.c_int64()
+    var = VARIANTS[variant] if isinstance(variant, str) else variant
+    rc = _lib().dmf_static_recalculate_noalloc(C.byref(c), len(src), _ptr(src), _ptr(dst), _ptr(w), s, t, kc, var,
+                                              C.c_void_p(stream_ptr) if stream_ptr else None, C.byref(f))
+    return rc, f.value
+
+
+def static_recalculate_noalloc(g: Graph, src, dst, w, s: int, t: int, kc: int, variant="data") -> int:
+    rc, f = static_recalculate_noalloc_status(g, src, dst, w, s, t, kc, variant)
+    _check(rc, "static_recalculate_noalloc")
+    return f

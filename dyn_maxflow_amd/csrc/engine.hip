@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -1104,6 +1105,127 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   return DMF_OK;
 }
 
+// ------------------------------------------------ allocation-free one-shot solve
+// dmf_static_recalculate_noalloc keeps the whole residual state of one graph of
+// at most kNaMaxN vertices and kNaMaxM arcs, and kNaMaxK update records, in
+// `scratch`, a fixed device array, instead of device memory it allocates.  The
+// array is a host-side stand-in in host code: its device address is taken
+// once per device with hipGetSymbolAddress and only that address is used.
+constexpr int32_t kNaMaxN = 1 << 16;
+constexpr int32_t kNaMaxM = 1 << 20;
+constexpr int32_t kNaMaxK = 1 << 20;
+constexpr int kNaMaxDevices = 64;
+
+constexpr size_t na_align(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+// Byte offsets of every array of the state inside the scratch array.
+struct NaLayout {
+  size_t row, col, rev, cap, res, rres, owner, excess, height, order, wl, wl2, part, side, pfinal, term, lev16,
+      widebits, wq_vert, wq_base, wide_vert, wide_base, wave_vert, wave_base, ctrl, ctrs, iflow, srcs, snks, usrc,
+      udst, uw, uarc, packed, total;
+};
+
+constexpr NaLayout na_layout() {
+  NaLayout L{};
+  size_t o = 0;
+  auto take = [&o](size_t bytes) {
+    const size_t at = o;
+    o += na_align(bytes);
+    return at;
+  };
+  const size_t N = kNaMaxN, M = kNaMaxM, K = kNaMaxK;
+  L.row = take(4 * (N + 1));
+  L.col = take(4 * M);
+  L.rev = take(4 * M);
+  L.cap = take(4 * M);
+  L.res = take(4 * M);
+  L.rres = take(4 * M);
+  L.owner = take(4 * M);
+  L.excess = take(8 * N);
+  L.height = take(4 * N);
+  L.order = take(4 * N);
+  L.wl = take(4 * N);
+  L.wl2 = take(4 * N);
+  L.part = take(4 * N);
+  L.side = take(N);
+  L.pfinal = take(N);
+  L.term = take(N);
+  L.lev16 = take(2 * N);
+  L.widebits = take(4 * (N / 32 + 1));
+  L.wq_vert = take(4 * kNumQ * N);
+  L.wq_base = take(4 * kNumQ * N);
+  L.wide_vert = take(4 * N);
+  L.wide_base = take(4 * N);
+  L.wave_vert = take(4 * N);
+  L.wave_base = take(4 * N);
+  L.ctrl = take(sizeof(Ctrl));
+  L.ctrs = take(8 * C_N * kSlots);
+  L.iflow = take(8);
+  L.srcs = take(4);
+  L.snks = take(4);
+  L.usrc = take(4 * K);
+  L.udst = take(4 * K);
+  L.uw = take(4 * K);
+  L.uarc = take(4 * K);
+  L.packed = take(16);
+  L.total = o;
+  return L;
+}
+constexpr NaLayout kNaLayout = na_layout();
+
+}  // namespace
+
+// The fixed workspace (about 46 MiB): held by every GPU the library runs on
+// until the process ends.  Calls on one GPU share it, so calls that overlap on
+// one GPU race; each call initialises what it reads.
+static __device__ __align__(256) unsigned char scratch[kNaLayout.total];
+
+namespace {
+
+std::mutex g_na_mutex;
+void* g_na_ptr[kNaMaxDevices] = {};
+
+// Device address of `scratch` on the current device, taken on the first call
+// on that device.
+int na_scratch(void** out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) return DMF_EDEVICE;
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (!g_na_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      std::fprintf(stderr, "dmf: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return DMF_EDEVICE;
+    }
+    g_na_ptr[dev] = ptr;
+  }
+  *out = g_na_ptr[dev];
+  return DMF_OK;
+}
+
+// dmf_csr_check's rules in place, without its threads and buffers.
+int na_check(const dmf_csr* g) {
+  if (!g || g->n < 0 || g->m < 0 || !g->row || (g->m > 0 && (!g->col || !g->cap || !g->rev))) return DMF_EINVAL;
+  const int32_t n = g->n, m = g->m;
+  if (g->row[0] != 0 || g->row[n] != m) return DMF_EFORMAT;
+  for (int32_t v = 0; v < n; ++v) {
+    if (g->row[v] > g->row[v + 1]) return DMF_EFORMAT;
+    for (int32_t e = g->row[v]; e < g->row[v + 1]; ++e) {
+      const int32_t h = g->col[e], r = g->rev[e];
+      if (h < 0 || h >= n || g->cap[e] < 0 || r < 0 || r >= m) return DMF_EFORMAT;
+      if (g->rev[r] != e || g->col[r] != v) return DMF_EFORMAT;
+    }
+  }
+  return DMF_OK;
+}
+
+template <class T>
+T* na_at(void* base, size_t off) {
+  return reinterpret_cast<T*>(static_cast<unsigned char*>(base) + off);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1606,6 +1728,161 @@ int dmf_static_recalculate(const dmf_csr* g, int32_t k, const int32_t* src, cons
     if (!rc && flow) *flow = f;
     dmf_engine_destroy(E);
     return rc;
+  });
+}
+
+int dmf_static_recalculate_noalloc(const dmf_csr* g, int32_t k, const int32_t* src, const int32_t* dst,
+                                   const int32_t* w, int32_t s, int32_t t, int32_t kc, int32_t variant,
+                                   void* hip_stream, int64_t* flow) {
+  return guarded([&]() -> int {
+    if (!g || !flow || k < 0 || (k > 0 && (!src || !dst || !w)) || variant < 0 || variant > 2) return DMF_EINVAL;
+    if (g->n > kNaMaxN || g->m > kNaMaxM || k > kNaMaxK) {
+      std::fprintf(stderr,
+                   "dmf: dmf_static_recalculate_noalloc takes at most %d vertices, %d arcs and %d updates "
+                   "(got %d, %d, %d)\n",
+                   kNaMaxN, kNaMaxM, kNaMaxK, g->n, g->m, k);
+      return DMF_ESIZE;
+    }
+    if (g->n < 2 || s < 0 || s >= g->n || t < 0 || t >= g->n || s == t) return DMF_EINVAL;
+    if (const int rc = na_check(g)) return rc;
+    void* base = nullptr;
+    if (const int rc = na_scratch(&base)) return rc;
+    const NaLayout& L = kNaLayout;
+    const int32_t n = g->n, m = g->m;
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    // an engine on the stack over the workspace: no stream, event, pinned
+    // buffer or graph of its own; the stepped driver only
+    dmf_engine E;
+    E.n = n;
+    E.m = m;
+    E.s = s;
+    E.t = t;
+    E.stream = st;
+    Ctrl hc{};
+    E.h_ctrl = &hc;
+    int dev = 0;
+    hipDeviceProp_t prop{};
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    E.num_cu = prop.multiProcessorCount;
+    E.grid_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E.num_cu)) * kGridBlocksPerCu));
+    Dev& D = E.D;
+    D.n = n;
+    D.m = m;
+    D.s = s;
+    D.t = t;
+    D.nsrc = D.nsnk = 1;
+    D.ninst = 1;
+    D.lane_max = kLaneMax;
+    D.wide_t = kWideT;
+    D.row = na_at<int32_t>(base, L.row);
+    D.col = na_at<int32_t>(base, L.col);
+    D.rev = na_at<int32_t>(base, L.rev);
+    D.cap = na_at<int32_t>(base, L.cap);
+    D.res = na_at<int32_t>(base, L.res);
+    D.rres = na_at<int32_t>(base, L.rres);
+    D.owner = na_at<int32_t>(base, L.owner);
+    D.excess = na_at<long long>(base, L.excess);
+    D.height = na_at<int32_t>(base, L.height);
+    D.order = na_at<int32_t>(base, L.order);
+    D.wl = na_at<int32_t>(base, L.wl);
+    D.wl2 = na_at<int32_t>(base, L.wl2);
+    D.part = na_at<int32_t>(base, L.part);
+    D.side = na_at<uint8_t>(base, L.side);
+    D.pfinal = na_at<uint8_t>(base, L.pfinal);
+    uint8_t* term = na_at<uint8_t>(base, L.term);
+    D.term = term;
+    D.lev16 = na_at<uint16_t>(base, L.lev16);
+    uint32_t* widebits = na_at<uint32_t>(base, L.widebits);
+    D.widebits = widebits;
+    D.wq_vert = na_at<int32_t>(base, L.wq_vert);
+    D.wq_base = na_at<int32_t>(base, L.wq_base);
+    D.wq_cap = n;
+    int32_t* wide_vert = na_at<int32_t>(base, L.wide_vert);
+    int32_t* wide_base = na_at<int32_t>(base, L.wide_base);
+    int32_t* wave_vert = na_at<int32_t>(base, L.wave_vert);
+    int32_t* wave_base = na_at<int32_t>(base, L.wave_base);
+    D.wide_vert = wide_vert;
+    D.wide_base = wide_base;
+    D.wave_vert = wave_vert;
+    D.wave_base = wave_base;
+    D.ctrl = na_at<Ctrl>(base, L.ctrl);
+    D.ctrs = na_at<unsigned long long>(base, L.ctrs);
+    D.iflow = na_at<long long>(base, L.iflow);
+    int32_t* srcs = na_at<int32_t>(base, L.srcs);
+    int32_t* snks = na_at<int32_t>(base, L.snks);
+    D.srcs = srcs;
+    D.snks = snks;
+    D.hkey = nullptr;  // arcs are found by scanning the tail's adjacency
+    D.hval = nullptr;
+    D.hmask = 0;
+    int32_t* usrc = na_at<int32_t>(base, L.usrc);
+    int32_t* udst = na_at<int32_t>(base, L.udst);
+    int32_t* uw = na_at<int32_t>(base, L.uw);
+    D.usrc = usrc;
+    D.udst = udst;
+    D.uw = uw;
+    D.uarc = na_at<int32_t>(base, L.uarc);
+    unsigned long long* packed = na_at<unsigned long long>(base, L.packed);
+    const int32_t s1 = s, t1 = t;
+    HIP_OK(hipMemcpyAsync(const_cast<int32_t*>(D.row), g->row, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (m > 0) {
+      HIP_OK(hipMemcpyAsync(const_cast<int32_t*>(D.col), g->col, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(const_cast<int32_t*>(D.rev), g->rev, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(D.cap, g->cap, sizeof(int32_t) * m, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemsetAsync(D.owner, 0xff, sizeof(int32_t) * m, st));
+    }
+    if (k > 0) {
+      HIP_OK(hipMemcpyAsync(usrc, src, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(udst, dst, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(uw, w, sizeof(int32_t) * k, hipMemcpyHostToDevice, st));
+    }
+    HIP_OK(hipMemcpyAsync(srcs, &s1, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(snks, &t1, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemsetAsync(term, 0, n, st));
+    HIP_OK(hipMemsetAsync(term + s, 1, 1, st));
+    HIP_OK(hipMemsetAsync(term + t, 2, 1, st));
+    HIP_OK(hipMemsetAsync(D.side, 0, n, st));
+    HIP_OK(hipMemsetAsync(D.pfinal, 0, n, st));
+    HIP_OK(hipMemsetAsync(widebits, 0, sizeof(uint32_t) * (n / 32 + 1), st));
+    HIP_OK(hipMemsetAsync(packed, 0, 16, st));
+    HIP_OK(hipMemsetAsync(D.ctrl, 0, sizeof(Ctrl), st));
+    HIP_OK(hipMemsetAsync(D.ctrs, 0, sizeof(unsigned long long) * C_N * kSlots, st));
+    hipLaunchKernelGGL(k_build_static_lists, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, wide_vert, wide_base,
+                       wave_vert, wave_base, widebits, packed);
+    HIP_OK(hipGetLastError());
+    unsigned long long words[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(words, packed, sizeof(words), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    D.wide_packed = words[0];
+    D.wave_packed = words[1];
+    // update_kernel (code/dyn_data_bench.cu:300-322): capacities from the records
+    if (k > 0) {
+      D.k = k;
+      hipLaunchKernelGGL(k_upd_lookup, dim3(blocks_for(k)), dim3(kBlock), 0, st, D);
+      hipLaunchKernelGGL(k_upd_apply, dim3(blocks_for(k)), dim3(kBlock), 0, st, D, 1, 0);
+      hipLaunchKernelGGL(k_upd_release, dim3(blocks_for(k)), dim3(kBlock), 0, st, D);
+      HIP_OK(hipGetLastError());
+      if (const int rc = sync_ctrl(&E)) return rc;
+      if (hc.missing) return DMF_ENOARC;
+      if (hc.invalid) return DMF_EINVAL;
+      D.k = 0;
+      HIP_OK(hipMemsetAsync(D.ctrl, 0, sizeof(Ctrl), st));
+    }
+    dmf_params p;
+    dmf_params_default(g, &p);
+    p.kc = kc > 0 ? kc : p.kc;
+    p.variant = variant;
+    p.mode = 1;
+    const Cfg C = make_cfg(&E, &p, 0);
+    int rc = solve_stepped(&E, C);
+    if (rc == DMF_EDEVICE) return rc;
+    const int rc_sync = sync_ctrl(&E);
+    if (rc_sync) return rc_sync;
+    if (rc) return rc;
+    if (hc.status) return hc.status;
+    *flow = hc.flow;
+    return DMF_OK;
   });
 }
 

@@ -153,6 +153,7 @@ const char* dmf_strerror(int status) {
     case DMF_EFORMAT: return "malformed file or CSR";
     case DMF_EADMIT: return "request exceeds the admission budget";
     case DMF_EAGAIN: return "result not ready";
+    case DMF_ESIZE: return "input exceeds the fixed workspace";
     default: return "unknown status";
   }
 }

@@ -395,7 +395,8 @@ __device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
 // atomic for the whole wave: slots by ballot rank, chunk bases by an exclusive
 // prefix sum of the lanes' chunk counts (increasing with the slot, as
 // wave_find_slot requires).  All 64 lanes.
-__device__ __forceinline__ void wave_wq_append(const Dev& D, int q, bool pred, int v, int deg) {
+__device__ __forceinline__ void wave_list_append(unsigned long long* word, int32_t* vert, int32_t* base, bool pred,
+                                                 int v, int deg) {
   const unsigned long long b = __ballot(pred);
   if (b == 0) return;
   const int c = pred ? (deg + kChunk - 1) / kChunk : 0;
@@ -403,13 +404,17 @@ __device__ __forceinline__ void wave_wq_append(const Dev& D, int q, bool pred, i
   const int tot = __shfl(incl, kWave - 1);
   unsigned long long old = 0;
   if (lane_id() == 0)
-    old = atomicAdd(&D.ctrl->wq[q], (static_cast<unsigned long long>(__popcll(b)) << 32) | static_cast<unsigned>(tot));
+    old = atomicAdd(word, (static_cast<unsigned long long>(__popcll(b)) << 32) | static_cast<unsigned>(tot));
   old = __shfl(old, 0);
   if (pred) {
     const size_t slot = static_cast<size_t>(old >> 32) + __popcll(b & ((1ull << lane_id()) - 1ull));
-    D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
-    D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu) + incl - c;
+    vert[slot] = v;
+    base[slot] = static_cast<int>(old & 0xffffffffu) + incl - c;
   }
+}
+__device__ __forceinline__ void wave_wq_append(const Dev& D, int q, bool pred, int v, int deg) {
+  wave_list_append(&D.ctrl->wq[q], D.wq_vert + static_cast<size_t>(q) * D.wq_cap,
+                   D.wq_base + static_cast<size_t>(q) * D.wq_cap, pred, v, deg);
 }
 
 // Lanes with pred set note wide vertex v (degree deg) into the block stage.
@@ -1534,6 +1539,14 @@ __device__ __forceinline__ unsigned long long pair_key(int u, int v) {
 }
 
 __device__ int lookup_arc(const Dev& D, int u, int v) {
+  if (!D.hkey) {
+    // no arc index (the allocation-free solve): scan u's arcs, the last match
+    // wins as in OnAdd_kernel's scan (code/dyn_data.h:191-201)
+    int found = -1;
+    for (int e = D.row[u]; e < D.row[u + 1]; ++e)
+      if (D.col[e] == v) found = e;
+    return found;
+  }
   const unsigned long long key = pair_key(u, v);
   unsigned long long h = hash_key(key) & D.hmask;
   for (;;) {
@@ -1852,6 +1865,25 @@ __global__ void __launch_bounds__(kBlock) k_build_imbalance_wl(Dev D, int cnt, i
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_count_imbalance(Dev D) { ph_count_imbalance(D, make_range()); }
+
+// Static lists of the vertices above the degree thresholds (the wide list of
+// degree > wide_t with its bitmap, and the wave list of degree > min(lane_max,
+// wide_t)), built on the device for engines that have no host copy of the CSR.
+// packed[0] / packed[1] receive the wide / wave list words (slots << 32 |
+// chunks); both and widebits must be zero on entry.
+__global__ void __launch_bounds__(kBlock) k_build_static_lists(Dev D, int32_t* wide_vert, int32_t* wide_base,
+                                                               int32_t* wave_vert, int32_t* wave_base,
+                                                               uint32_t* widebits, unsigned long long* packed) {
+  const Range r = make_range();
+  const int wave_t = min(D.lane_max, D.wide_t);
+  for (int base = r.gid - lane_id(); base < D.n; base += r.gsz) {
+    const int v = base + lane_id();
+    const int d = v < D.n ? degree(D, v) : 0;
+    wave_list_append(&packed[0], wide_vert, wide_base, d > D.wide_t, v, d);
+    wave_list_append(&packed[1], wave_vert, wave_base, d > wave_t, v, d);
+    if (v < D.n && d > D.wide_t) atomicOr(&widebits[static_cast<unsigned>(v) >> 5], 1u << (v & 31));
+  }
+}
 
 // Builds the (tail, head) -> arc hash index, one thread per arc; for a repeated
 // (tail, head) pair the highest arc index wins, as the reference's last-match scan.

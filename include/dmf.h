@@ -29,6 +29,7 @@ extern "C" {
 #define DMF_EFORMAT (-7)  /* file or CSR inconsistent (negative sizes, ...)     */
 #define DMF_EADMIT (-8)   /* serving: request exceeds the admission budget      */
 #define DMF_EAGAIN (-9)   /* serving: result not ready within the timeout       */
+#define DMF_ESIZE (-10)   /* input larger than a fixed workspace allows         */
 
 /* Human-readable text of a status code (static storage). */
 const char* dmf_strerror(int status);
@@ -371,6 +372,23 @@ int dmf_recalculate_max_flow(const dmf_csr* g, int32_t k, const int32_t* src,
 int dmf_static_recalculate(const dmf_csr* g, int32_t k, const int32_t* src,
                            const int32_t* dst, const int32_t* w, int32_t s, int32_t t,
                            int32_t kc, int32_t variant, int64_t* flow, double* ms);
+
+/* dmf_static_recalculate without any allocation: no reference counterpart.
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc): the residual state lives in a fixed
+ * workspace of about 46 MiB compiled into the library, which every GPU the
+ * library runs on holds until the process ends.  Graphs of at most 65,536
+ * vertices and 1,048,576 arcs and at most 1,048,576 update records; larger
+ * inputs fail with DMF_ESIZE before any device work.  The caller passes host
+ * arrays as for dmf_static_recalculate and a HIP stream (NULL = the default
+ * stream); the solve runs on the stepped driver with the fixed schedule and
+ * returns the same flow.  Calls on one GPU share the workspace: calls that
+ * overlap on one GPU race, so serialise them.  No device memory, stream, event,
+ * pinned buffer or graph is created. */
+int dmf_static_recalculate_noalloc(const dmf_csr* g, int32_t k, const int32_t* src,
+                                   const int32_t* dst, const int32_t* w, int32_t s,
+                                   int32_t t, int32_t kc, int32_t variant, void* hip_stream,
+                                   int64_t* flow);
 
 /* Number of HIP devices visible (0 on a machine without a GPU). */
 int dmf_device_count(void);

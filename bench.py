@@ -4,13 +4,22 @@ Workload (one GPU): directed R-MAT scale 20, edge factor 16, a/b/c/d =
 0.57/0.19/0.19/0.05, self-loops and duplicates dropped, seed 2, capacities
 uniform [1,100]; s = max out-degree vertex, t = max in-degree vertex != s.
 One repeat = reset to the original graph, static solve (untimed; push-pull by
-default, for the plain and the learned runs alike, so that the scheduler may
-send a batch to any variant), then 10 successive update batches of 1%, 2%,
-..., 10% of the input edges (50/50 increase/decrease).  A step is one batch: apply + repair + re-converge, timed
-with HIP events on the engine's stream, inputs already in HBM.
+default, for every pass alike, so that the scheduler may send a batch to any
+variant), then 10 successive update batches of 1%, 2%, ..., 10% of the input
+edges (50/50 increase/decrease).  A step is one batch: apply + repair +
+re-converge, timed with HIP events on the engine's stream, inputs already in
+HBM.
 
-Prints one JSON line (see the keys at the bottom).  --gpus > 1 is not
-supported on this build (one GPU): it exits with a message.
+Passes: the learned scheduler (headline `value`), the plain fixed schedule of
+the reference for the data-driven, push-pull and topology-driven variants
+(global relabel every round), and a profiled plain pass that times every BFS
+level launch of the default driver for the roofline.  After the timed runs
+every batch's flow and S_min are checked against the oracle's Dinic, and the
+oracle's CPU push-relabel is timed on a bounded sample (single-threaded and
+with OpenMP).
+
+Prints one JSON line (see build_line).  --gpus > 1 is not supported on this
+build (one GPU): it exits with a message.
 """
 from __future__ import annotations
 
@@ -30,7 +39,9 @@ BYTES_ARC, BYTES_VERTEX, BYTES_UPDATE = 12, 16, 24
 # HBM traffic of the dominant kernel measured with rocprofv3 --pmc (FETCH_SIZE,
 # WRITE_SIZE per dispatch, scaled by a plain copy of known size); written from
 # the committed profile summary of this workload
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bfs_level_rmat20_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bfs_level_dev_rmat20_traffic.json")
+DRIVERS = {0: "persistent", 1: "stepped", 2: "stepped, BFS levels replayed from hipGraphs"}
+LEVEL_KERNEL = {0: "k_solve_persistent", 1: "k_bfs_level", 2: "k_bfs_level_dev"}
 
 
 def log(*a):
@@ -99,10 +110,13 @@ class HipBuffers:
         self.ptrs = []
 
 
-def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=None, static_params=None):
+def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=None, static_params=None,
+            sides=None):
     """Returns per-batch records for `steps` timed batches after `warmup` untimed.
     With profile, every BFS level launch is timed by HIP events and the BFS
-    counters of every solve (static ones included) are summed into prof."""
+    counters of every solve (static ones included) are summed into prof.
+    sides (a dict) receives S_min after each batch of the first repeat, read
+    outside the timed span."""
     import dyn_maxflow_amd as dmf
     bufs = HipBuffers()
     dev = [tuple(bufs.upload(a) for a in b) for b in batches]
@@ -133,6 +147,8 @@ def run_gpu(g, s, t, batches, params, steps, warmup, label, profile=False, prof=
                 if bi in flows and flows[bi] != f:
                     raise RuntimeError(f"{label}: batch {bi} flow {f} differs from earlier repeat {flows[bi]}")
                 flows[bi] = f
+                if sides is not None and bi not in sides:
+                    sides[bi] = eng.min_cut()
                 if done >= warmup:
                     c["wall_ms"] = wall
                     c["batch"] = bi
@@ -169,57 +185,86 @@ def summarize(recs):
     }
 
 
-def verify(g, s, t, batches, flows, which):
-    """Flow after successive batches vs the oracle's Dinic (CPU)."""
+def pass_object(S):
+    """The per-pass object of the JSON line."""
+    return {"p50_ms": round(S["p50_ms"], 4), "p99_ms": round(S["p99_ms"], 4), "mean_ms": round(S["mean_ms"], 4),
+            "gteps": round(S["gteps"], 4), "rounds_per_batch": S["rounds_mean"],
+            "global_relabels_per_batch": S["global_relabels_mean"],
+            "launches_per_batch": S["launches_per_batch"],
+            "achieved_gbs": round(S["achieved_gbs"], 2),
+            "roofline_frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "atomics_per_batch": round(S["atomics_per_batch"]),
+            "atomic_gops": round(S["atomic_gops"], 4)}
+
+
+def caps_sequence(g, batches):
+    """Capacities after each successive batch (last record wins)."""
+    key = g.tails().astype(np.int64) * g.n + g.col
+    order = np.argsort(key, kind="stable")
+    sk = key[order]
+    cap = g.cap.copy()
+    out = []
+    for src, dst, w in batches:
+        keep = src != dst
+        k = src[keep].astype(np.int64) * g.n + dst[keep]
+        pos = order[np.searchsorted(sk, k, side="right") - 1]
+        cap[pos] = w[keep]
+        out.append(cap.copy())
+    return out
+
+
+def verify(g, s, t, batches, flows, sides):
+    """Flow and S_min after every successive batch vs the oracle's Dinic (CPU,
+    one thread per batch)."""
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    tails = g.tails()
-    key = tails.astype(np.int64) * g.n + g.col
-    order = np.argsort(key)
-    cap = g.cap.copy()
+    caps = caps_sequence(g, batches)
+    with ThreadPoolExecutor(max_workers=min(len(caps), 6)) as ex:
+        ref = list(ex.map(lambda c: oracle.dinic(g, s, t, cap=c), caps))
     ok = True
-    for bi, (src, dst, w) in enumerate(batches):
-        k = src.astype(np.int64) * g.n + dst
-        pos = order[np.searchsorted(key[order], k)]
-        cap[pos] = w
-        if bi in which:
-            fd, _ = oracle.dinic(g, s, t, cap=cap)
-            ok = ok and (fd == flows[bi])
-            log(f"[bench] verify batch {bi}: gpu {flows[bi]} dinic {fd}")
+    for bi, (fd, side) in enumerate(ref):
+        same_cut = bi in sides and np.array_equal(sides[bi], side)
+        ok = ok and fd == flows[bi] and same_cut
+        log(f"[bench] verify batch {bi + 1}: gpu {flows[bi]} dinic {fd} S_min equal {same_cut}")
     return ok
 
 
-def cpu_baseline(g, s, t, batches, kc, seconds_cap):
+def cpu_baseline(g, s, t, batches, kc, seconds_cap, threads):
+    """The oracle's push-relabel with the same update repair, data variant, on
+    this host: a bounded sample of the same successive batches, timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, os.cpu_count() or 1)
     pr = oracle.PushRelabel(g, s, t, threads=threads)
-    pr.static("data", kc)
-    times = []
-    t_start = time.time()
-    for bi, (src, dst, w) in enumerate(batches):
-        t0 = time.perf_counter()
-        pr.apply(src, dst, w, "data", kc)
-        times.append((time.perf_counter() - t0) * 1e3)
-        if time.time() - t_start > seconds_cap:
-            break
+    try:
+        pr.static("data", kc)
+        times = []
+        t_start = time.time()
+        for src, dst, w in batches:
+            t0 = time.perf_counter()
+            pr.apply(src, dst, w, "data", kc)
+            times.append((time.perf_counter() - t0) * 1e3)
+            if time.time() - t_start > seconds_cap:
+                break
+    finally:
+        pr.close()
     return {
-        "value": float(np.mean(times)),
+        "value": round(float(np.mean(times)), 3),
         "unit": "ms/batch",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle push-relabel (data variant, OpenMP, kc={kc}) on the same RMAT-20 graph: "
-                  f"static solve untimed, then the first {len(times)} successive batches "
+        "sample": f"oracle push-relabel (data variant, {'OpenMP' if threads > 1 else 'sequential'}, kc={kc}) on the "
+                  f"same RMAT-20 graph: static solve untimed, then the first {len(times)} successive batches "
                   f"({', '.join(str(i + 1) + '%' for i in range(len(times)))}) timed",
     }
 
 
-def roofline(prof):
-    """Dominant kernel (k_bfs_level, the global relabel's level expansion) of
-    the plain path: algorithmic bytes per launch (12 B per arc visit + 16 B per
-    vertex visit, device counters) / mean launch duration (HIP events stamped
-    by each launch on the engine's stream), over every level launch of the
-    profiled pass."""
+def roofline(prof, mode):
+    """Dominant kernel (the global relabel's level expansion) of the plain
+    path on the bench's driver: algorithmic bytes per launch (12 B per arc
+    visit + 16 B per vertex visit, device counters) / mean launch duration
+    (HIP events on the engine's stream around every graph-replayed segment of
+    level launches, or around every level launch on the stepped driver)."""
     n = prof.get("bfs_level_launches", 0)
     ms = prof.get("bfs_level_ms", 0.0)
     if not n or ms <= 0:
@@ -229,17 +274,22 @@ def roofline(prof):
     tr = traffic_from_profile()
     return {
         "bound": "hbm",
-        "kernel": "k_bfs_level",
+        "kernel": LEVEL_KERNEL[mode],
         "achieved": round(gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(gbs / HBM_PEAK_GBS, 4),
         "traffic": tr["traffic_bytes_per_launch"] if tr else None,
         "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if tr else None,
+        # L2 atomic requests per launch (rocprofv3 TCC_ATOMIC, same summary)
+        # over this pass's mean launch duration
+        "tcc_atomic_per_launch": tr.get("tcc_atomic_per_dispatch") if tr else None,
+        "atomic_gops": round(tr["tcc_atomic_per_dispatch"] / (ms / n) / 1e6, 4)
+        if tr and tr.get("tcc_atomic_per_dispatch") else None,
         "bytes_alg_per_launch": round(bytes_alg / n),
         "avg_launch_us": round(ms * 1e3 / n, 2),
         "launches": n,
-        "pass": "profiled plain pass on the stepped driver: static solves and batches, every level launch",
+        "pass": f"profiled plain pass on the {DRIVERS[mode]} driver: static solves and batches, every level launch",
     }
 
 
@@ -259,12 +309,61 @@ def copy_probe(mib):
 
 
 def traffic_from_profile():
-    """HBM bytes per k_bfs_level launch from the committed PMC summary, or None."""
+    """HBM bytes per level launch from the committed PMC summary, or None."""
     try:
         with open(TRAFFIC_FILE) as fh:
             return json.load(fh)
     except (OSError, ValueError):
         return None
+
+
+def build_line(args, g, s, t, kc, S, L, variants, roof, cpu, cpu1, verified):
+    """The one JSON line bench.py prints.  S: plain data-driven summary, L:
+    learned summary (or None), variants: {name: summary} of the plain runs."""
+    head = L if L is not None else S
+    out = {
+        "metric": "p50 ms per update batch (RMAT-20, successive 1-10% mixed batches)",
+        "value": round(head["p50_ms"], 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["mean_ms"], 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32/int64",
+        "data": "synthetic (seeded R-MAT generator)",
+        "config": {
+            "workload": f"rmat-{args.scale} ef{args.ef} seed {args.seed}, 10 successive batches 1%..10% "
+                        "of the input edges, 50/50 inc/dec, caps U[1,100]",
+            "n": g.n, "m": g.m, "s": s, "t": t,
+            "variant": "data", "kc": kc, "relabel_interval": args.interval,
+            "static_solve_variant": args.static_variant,
+            "driver": DRIVERS[args.mode],
+        },
+        "scheduler": "learned" if L is not None else "fixed",
+        "p99_ms": round(head["p99_ms"], 4),
+        "wall_p50_ms": round(head["wall_p50_ms"], 4),
+        "gteps": round(head["gteps"], 4),
+        "rounds_per_batch": head["rounds_mean"],
+        "atomics_per_batch": round(head["atomics_per_batch"]),
+        "atomic_gops": round(head["atomic_gops"], 4),
+        "plain": pass_object(S),
+        "learned": pass_object(L) if L is not None else None,
+        "plain_variants": {k: pass_object(v) for k, v in variants.items()},
+        "learned_speedup_p50": round(S["p50_ms"] / L["p50_ms"], 3) if L is not None else None,
+        "flows_and_smin_verified_vs_dinic": verified,
+        "roofline": roof,
+        "all_kernels": {"achieved_gbs": round(S["achieved_gbs"], 2),
+                        "frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                        "bytes_alg_per_batch": round(S["bytes_per_batch"])},
+        "cpu_baseline": cpu,
+        "cpu_baseline_single_thread": cpu1,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = dict(cpu, host_nproc=os.cpu_count())
+    return out
 
 
 def main():
@@ -275,22 +374,26 @@ def main():
     ap.add_argument("--scale", type=int, default=20)
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--variant", default="data")
     ap.add_argument("--mode", type=int, default=2,
                     help="driver: 0 persistent one-launch solve, 1 stepped, 2 stepped with graph-replayed BFS levels")
-    ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels")
+    ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels of the plain passes")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")
+    ap.add_argument("--variants", default="data,pp,topo",
+                    help="variants of the plain fixed-schedule passes (the first is the headline's plain)")
     ap.add_argument("--static-variant", default="pp",
                     help="variant of the untimed static solve that every repeat starts from (the same for "
-                         "the plain and the learned runs; push-pull lets every variant take the batches)")
+                         "every pass; push-pull lets every variant take the batches)")
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--roofline-steps", type=int, default=30,
                     help="batches of the profiled plain pass that measures the dominant kernel")
     ap.add_argument("--roofline-only", action="store_true",
-                    help="run only the profiled pass (the command its rocprofv3 summary is taken from)")
+                    help="run only the profiled pass (the command its rocprofv3 summaries are taken from)")
+    ap.add_argument("--only", default=None,
+                    help="run only the plain pass of this variant (data, pp or topo) and print its object "
+                         "(the command the per-variant rocprofv3 summaries are taken from)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (PMC scaling reference) and exit")
     args = ap.parse_args()
@@ -308,76 +411,53 @@ def main():
         return
     g, s, t, _, batches = make_workload(args.scale, args.ef, args.seed, 10)
     kc = max(1, g.m // g.n)
-    plain = dmf.default_params(g, variant=args.variant, mode=args.mode, relabel_interval=args.interval)
     static = dmf.default_params(g, variant=args.static_variant, mode=args.mode)
-    # The roofline pass runs the plain schedule on the stepped driver: its BFS
-    # level launches (k_bfs_level, the same level expansion that the graph
-    # driver replays) can be timed one by one with launch-stamped events.
+
+    def plain_params(variant):
+        return dmf.default_params(g, variant=variant, mode=args.mode, relabel_interval=args.interval)
+
+    if args.only:
+        recs, _ = run_gpu(g, s, t, batches, plain_params(args.only), args.steps, args.warmup, f"plain {args.only}",
+                          static_params=static)
+        print(json.dumps({"variant": args.only, "plain": pass_object(summarize(recs))}), flush=True)
+        return
+    # Profiled pass: the plain schedule on the bench's driver, with every level
+    # launch (graph driver: every replayed segment of them) timed by HIP events.
     prof = {}
-    plain_prof = dmf.default_params(g, variant=args.variant, mode=1, relabel_interval=args.interval)
-    precs, _ = run_gpu(g, s, t, batches, plain_prof, args.roofline_steps, 2, "profiled plain", profile=True,
-                       prof=prof, static_params=dmf.default_params(g, variant=args.static_variant, mode=1))
-    roof = roofline(prof)
+    precs, _ = run_gpu(g, s, t, batches, plain_params("data"), args.roofline_steps, 2, "profiled plain",
+                       profile=True, prof=prof, static_params=static)
+    roof = roofline(prof, args.mode)
     if args.roofline_only:
         print(json.dumps({"roofline": roof, "profiled_p50_ms": nearest_rank([r["device_ms"] for r in precs], 50)}),
               flush=True)
         return
-    recs, flows = run_gpu(g, s, t, batches, plain, args.steps, args.warmup, "plain", static_params=static)
-    S = summarize(recs)
+    names = [v for v in args.variants.split(",") if v]
+    sides = {}
+    variants, flows = {}, None
+    for vi, var in enumerate(names):
+        recs, fl = run_gpu(g, s, t, batches, plain_params(var), args.steps, args.warmup, f"plain {var}",
+                           static_params=static, sides=sides if vi == 0 else None)
+        variants[var] = summarize(recs)
+        if flows is None:
+            flows = fl
+        elif fl != flows:
+            raise RuntimeError(f"plain {var} produced different flows from plain {names[0]}")
+    S = variants[names[0]]
     L = None
     if args.sched != "plain":
-        learned = dmf.default_params(g, variant=args.variant, mode=args.mode if args.mode else 1,
-                                     scheduler=dmf.SCHED_LEARNED)
-        lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned",
-                                static_params=static)
+        learned = dmf.default_params(g, mode=args.mode if args.mode else 1, scheduler=dmf.SCHED_LEARNED)
+        lrecs, lflows = run_gpu(g, s, t, batches, learned, args.steps, args.warmup, "learned", static_params=static)
         L = summarize(lrecs)
         if lflows != flows:
             raise RuntimeError("learned scheduler produced different flows from the plain path")
-    verified = None
-    if not args.no_verify:
-        verified = verify(g, s, t, batches, flows, which={0, len(batches) - 1})
-    cpu = None if args.no_cpu else cpu_baseline(g, s, t, batches, kc, args.cpu_seconds)
-
-    head = L if L is not None else S
-    out = {
-        "metric": "p50 ms per update batch (RMAT-20, successive 1-10% mixed batches)",
-        "value": round(head["p50_ms"], 4),
-        "unit": "ms",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(head["mean_ms"], 4),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32/int64",
-        "data": "synthetic (seeded R-MAT generator)",
-        "config": {
-            "workload": f"rmat-{args.scale} ef{args.ef} seed {args.seed}, 10 successive batches 1%..10% "
-                        f"of {g.m // 2 if False else 'input'} edges, 50/50 inc/dec, caps U[1,100]",
-            "n": g.n, "m": g.m, "s": s, "t": t,
-            "variant": args.variant, "kc": kc, "relabel_interval": args.interval,
-            "static_solve_variant": args.static_variant,
-            "driver": {0: "persistent", 1: "stepped", 2: "stepped, BFS levels replayed from hipGraphs"}[args.mode],
-        },
-        "scheduler": "learned" if L is not None else "fixed",
-        "p99_ms": round(head["p99_ms"], 4),
-        "wall_p50_ms": round(head["wall_p50_ms"], 4),
-        "gteps": round(head["gteps"], 4),
-        "rounds_per_batch": head["rounds_mean"],
-        "plain": {"p50_ms": round(S["p50_ms"], 4), "p99_ms": round(S["p99_ms"], 4),
-                  "gteps": round(S["gteps"], 4), "rounds_per_batch": S["rounds_mean"],
-                  "global_relabels_per_batch": S["global_relabels_mean"],
-                  "roofline_frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4)},
-        "learned_speedup_p50": round(S["p50_ms"] / L["p50_ms"], 3) if L is not None else None,
-        "flows_verified_vs_dinic": verified,
-        "roofline": roof,
-        "all_kernels": {"achieved_gbs": round(S["achieved_gbs"], 2),
-                        "frac": round(S["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                        "bytes_alg_per_batch": round(S["bytes_per_batch"])},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    verified = None if args.no_verify else verify(g, s, t, batches, flows, sides)
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        # the GPU box gives this process a 16-core share (OMP_NUM_THREADS)
+        threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(g, s, t, batches, kc, args.cpu_seconds, threads)
+        cpu1 = cpu_baseline(g, s, t, batches, kc, args.cpu_seconds, 1)
+    print(json.dumps(build_line(args, g, s, t, kc, S, L, variants, roof, cpu, cpu1, verified)), flush=True)
 
 
 if __name__ == "__main__":

@@ -190,8 +190,9 @@ int bfs_blocks(int64_t cnt, int64_t chunks) {
 int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
   Dev& D = E->D;
   int rc;
-  if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) return rc;
-  if ((rc = zero_dev(E, D.ctrl->wq, 3 * sizeof(unsigned long long)))) return rc;
+  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, nullptr);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
   else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode);
   int lo = 0, lvl = 0;
@@ -277,10 +278,7 @@ void destroy_segments(dmf_engine* E) {
 int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
   Dev& D = E->D;
   int rc;
-  if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) return rc;
-  if ((rc = zero_dev(E, D.ctrl->wq, 3 * sizeof(unsigned long long)))) return rc;
-  if ((rc = zero_dev(E, D.ctrl->bcur, sizeof(D.ctrl->bcur)))) return rc;
-  hipLaunchKernelGGL(k_store_dev, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev);
+  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev);
   ++E->launches;
   HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
@@ -341,12 +339,12 @@ double since_solve_ms(const dmf_engine* E) {
 
 // Evaluates the scheduler MLP on the device and reads its decision back.
 int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int round, int variant,
-              int relabelled) {
+              int relabelled, double active) {
   Dev& D = E->D;
   const int k = D.k;
   const float inc = k > 0 ? static_cast<float>(E->h_ctrl->inc) / static_cast<float>(k) : 0.5f;
   hipLaunchKernelGGL(k_sched, dim3(1), dim3(kWave), 0, E->stream, D, E->d_sched_w, E->d_sched, phase, k, inc,
-                     rounds_since, E->last_levels);
+                     rounds_since, E->last_levels, active);
   ++E->launches;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(E->h_sched, E->d_sched, sizeof(SchedState), hipMemcpyDeviceToHost, E->stream));
@@ -389,7 +387,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     if (consult) {
       // batch start: the scheduler picks the variant (push-pull only from a
       // push-pull static state)
-      if ((rc = run_sched(E, 0, 0, "batch", 0, C.variant, 0))) return rc;
+      if ((rc = run_sched(E, 0, 0, "batch", 0, C.variant, 0, 0.0))) return rc;
       if (learned && E->h_sched->valid && (E->h_sched->variant != DMF_VARIANT_PP || E->pp_ready))
         C.variant = E->h_sched->variant;
     }
@@ -409,7 +407,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   }
   int round = 0;
   bool conv = true;
-  if ((rc = reset_lists(E))) return rc;
+  double active = 0;  // vertices listed for the last round (scheduler feature)
   if (!pp || C.entry == 0) {
     int since = C.interval, visited = 0;
     bool want_relabel = true;  // scheduler decision for the coming round
@@ -434,7 +432,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       int32_t cnt = 0;
       unsigned long long wpk = 0;
       if (!topo) {
-        if ((rc = reset_lists(E))) return rc;
+        // the list counters were cleared by the last push launch (or k_begin)
         LAUNCH(k_build_wl, blocks_for(visited), D, visited);
         if ((rc = sync_ctrl(E))) return rc;
         cnt = E->h_ctrl->wl_cnt;
@@ -451,19 +449,21 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         break;
       }
       if (topo) {
-        if ((rc = zero_dev(E, &D.ctrl->flag[0], sizeof(int32_t)))) return rc;
-        LAUNCH(k_push, phase_blocks(n, static_chunks), D, 0, n, C.kc, PM_ALL, 0ull);
+        const int fs = round & 1;
+        LAUNCH(k_push, phase_blocks(n, static_chunks), D, 0, n, C.kc, PM_ALL, 0ull, fs);
         LAUNCH(k_repair, phase_blocks(n, static_chunks), D, 0, n, RM_ALL, 0ull);
         if ((rc = sync_ctrl(E))) return rc;
-        if (!E->h_ctrl->flag[0]) break;
+        if (!E->h_ctrl->flag[fs]) break;
+        active = static_cast<double>(n);
       } else {
-        LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk);
+        LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0);
         LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
+        active = static_cast<double>(cnt) + static_cast<double>(wpk >> 32);
       }
       ++round;
       ++since;
       if (consult) {
-        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0))) return rc;
+        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0, active))) return rc;
         want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
       }
     }
@@ -489,7 +489,6 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         since = 0;
         fresh = true;
       }
-      if ((rc = reset_lists(E))) return rc;
       LAUNCH(k_build_pp_wl, blocks_for(visited), D, visited);
       if ((rc = sync_ctrl(E))) return rc;
       const int np = E->h_ctrl->wl_cnt, nq = E->h_ctrl->wl2_cnt;
@@ -504,14 +503,15 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         conv = false;
         break;
       }
-      if (np || wp) LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp);
-      if (nq || wq) LAUNCH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq);
+      if (np || wp) LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
+      if (nq || wq) LAUNCH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+      active = static_cast<double>(np) + nq + static_cast<double>(wp >> 32) + static_cast<double>(wq >> 32);
       if (np || wp) LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
       if (nq || wq) LAUNCH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
       ++round;
       ++since;
       if (consult) {
-        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0))) return rc;
+        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0, active))) return rc;
         want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
       }
     }
@@ -524,8 +524,6 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         for (;;) {
           int visited = 0;
           if ((rc = relabel(E, BFS_FINAL, parts, &visited))) return rc;
-          if ((rc = zero_dev(E, &D.ctrl->wl_cnt, sizeof(int32_t)))) return rc;
-          if ((rc = zero_dev(E, &D.ctrl->wq[kQActive], sizeof(unsigned long long)))) return rc;
           LAUNCH(k_build_final_wl, blocks_for(parts), D, parts);
           if ((rc = sync_ctrl(E))) return rc;
           const int32_t cnt = E->h_ctrl->wl_cnt;
@@ -535,7 +533,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
             conv = false;
             break;
           }
-          LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk);
+          LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0);
           LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_PFINAL, wpk);
           ++round;
         }
@@ -543,10 +541,9 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       }
     }
   }
-  if ((rc = reset_lists(E))) return rc;
-  if ((rc = zero_dev(E, D.ctrl->flag, sizeof(D.ctrl->flag)))) return rc;
-  if ((rc = zero_dev(E, &D.ctrl->flow, sizeof(long long)))) return rc;
-  if ((rc = zero_dev(E, D.iflow, sizeof(long long) * D.ninst))) return rc;
+  hipLaunchKernelGGL(k_flow_prep, dim3(1), dim3(kBlock), 0, E->stream, D);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
   LAUNCH(k_flow, blocks_for(n), D);
   E->last.rounds = round;
   E->last_variant = C.variant;
@@ -597,10 +594,11 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
 
 // Zero the control block's counters before a solve.
 int begin_solve(dmf_engine* E) {
-  HIP_OK(hipMemsetAsync(E->D.ctrl, 0, sizeof(Ctrl), E->stream));
-  HIP_OK(hipMemsetAsync(E->D.ctrs, 0, sizeof(unsigned long long) * C_N * kSlots, E->stream));
   E->last = dmf_counters{};
   E->launches = 0;
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(kBlock), 0, E->stream, E->D);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
   return DMF_OK;
 }
 
@@ -697,8 +695,8 @@ int complete_phase(dmf_engine* E, bool deficits, int kc, int max_rounds) {
     const unsigned long long wpk = E->h_ctrl->wq[deficits ? kQPull : kQActive];
     if (cnt == 0 && wpk == 0) return DMF_OK;
     if (round >= max_rounds) return DMF_ENOCONV;
-    if (deficits) LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 2, cnt, kc, PM_NO_SRC, wpk);
-    else LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, kc, PM_NO_SNK, wpk);
+    if (deficits) LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 2, cnt, kc, PM_NO_SRC, wpk, 0);
+    else LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, kc, PM_NO_SNK, wpk, 0);
   }
 }
 
@@ -1438,8 +1436,7 @@ int dmf_engine_apply_batch(dmf_engine* E, int32_t k, const int32_t* src, const i
 int dmf_engine_flow(dmf_engine* E, int64_t* flow) {
   return guarded([&]() -> int {
     if (!E || !flow) return DMF_EINVAL;
-    HIP_OK(hipMemsetAsync(&E->D.ctrl->flow, 0, sizeof(long long), E->stream));
-    HIP_OK(hipMemsetAsync(E->D.iflow, 0, sizeof(long long) * E->D.ninst, E->stream));
+    hipLaunchKernelGGL(k_flow_prep, dim3(1), dim3(kBlock), 0, E->stream, E->D);
     hipLaunchKernelGGL(k_flow, dim3(blocks_for(E->n)), dim3(kBlock), 0, E->stream, E->D);
     HIP_OK(hipGetLastError());
     int rc = sync_ctrl(E);

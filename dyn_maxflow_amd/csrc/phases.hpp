@@ -1707,6 +1707,39 @@ __device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int32_
 }
 
 // ============================================================ stepped kernels
+// Control-block clears done by kernels rather than hipMemsetAsync: each is one
+// small launch in place of several runtime fill dispatches.
+// k_begin: the whole control block and the work counters, before a solve.
+__global__ void __launch_bounds__(kBlock) k_begin(Dev D) {
+  int32_t* c = reinterpret_cast<int32_t*>(D.ctrl);
+  for (int i = threadIdx.x; i < static_cast<int>(sizeof(Ctrl) / sizeof(int32_t)); i += kBlock) c[i] = 0;
+  for (int i = threadIdx.x; i < C_N * kSlots; i += kBlock) D.ctrs[i] = 0;
+}
+// k_relabel_prep: the frontier ring, the BFS wide queues and the replay
+// cursor, before a global relabel; with out set, also the device copy of Dev
+// that graph-replayed levels read.
+__global__ void k_relabel_prep(Dev D, Dev* out) {
+  if (threadIdx.x == 0) {
+    Ctrl* c = D.ctrl;
+    c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+    c->wq[0] = c->wq[1] = c->wq[2] = 0;
+    c->bcur[0][0] = c->bcur[0][1] = c->bcur[1][0] = c->bcur[1][1] = 0;
+    if (out) *out = D;
+  }
+}
+// k_flow_prep: list counters, flags, the flow and the per-instance flows,
+// before the flow reduction.
+__global__ void __launch_bounds__(kBlock) k_flow_prep(Dev D) {
+  Ctrl* c = D.ctrl;
+  if (threadIdx.x == 0) {
+    c->wl_cnt = c->wl2_cnt = c->part_cnt = 0;
+    c->wq[kQActive] = c->wq[kQPull] = 0;
+    c->flag[0] = c->flag[1] = 0;
+    c->flow = 0;
+  }
+  for (int i = threadIdx.x; i < D.ninst; i += kBlock) D.iflow[i] = 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_init_residual(Dev D) { ph_init_residual(D, make_range()); }
 __global__ void __launch_bounds__(kBlock) k_saturate_source(Dev D) {
   Tally t;
@@ -1752,11 +1785,8 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
 // frontier of the current level is found from the device cursor rather than
 // from kernel arguments, and once a level comes out empty every later launch
 // of the segment returns at once.  The engine state is read through a device
-// copy of Dev (k_store_dev) because captured kernel arguments are frozen at
-// capture time.
-__global__ void k_store_dev(Dev D, Dev* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = D;
-}
+// copy of Dev (written by k_relabel_prep) because captured kernel arguments
+// are frozen at capture time.
 // Reads cursor slot r.  Block 0 writes the next level's cursor into slot r^1,
 // which no block of this launch reads, and clears ring slot (lvl+2)%3 (the
 // size of level lvl-1, already folded into the cursor; level lvl+1 appends
@@ -1806,13 +1836,26 @@ __global__ void __launch_bounds__(kBlock) k_build_final_wl(Dev D, int cnt) {
   ph_build_final_wl(D, make_range(), cnt, t);
   t.flush(D.ctrs);
 }
-// which: 0 = all vertices (topology), 1 = wl, 2 = wl2 (pull)
+// which: 0 = all vertices (topology), 1 = wl, 2 = wl2 (pull).
+// The list sizes were read by the host before this launch and come in as
+// arguments, so block 0 clears the list counters for the next round's builder.
+// Topology-driven rounds alternate flag slots: this round reports activity in
+// flag[fslot] and clears flag[fslot ^ 1], which the host read a round ago.
 __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int kc, int pmode,
-                                                 unsigned long long wpk) {
+                                                 unsigned long long wpk, int fslot) {
   Tally t;
   const Range r = make_range();
+  if (is_block0_thread0()) {
+    Ctrl* c = D.ctrl;
+    if (which == 0) {
+      c->flag[fslot ^ 1] = 0;
+    } else {
+      c->wl_cnt = c->wl2_cnt = 0;
+      c->wq[kQActive] = c->wq[kQPull] = 0;
+    }
+  }
   if (which == 0) {
-    ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[0], wlist_waves(D), t);
+    ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[fslot], wlist_waves(D), t);
   } else if (which == 1) {
     ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t);
   } else {

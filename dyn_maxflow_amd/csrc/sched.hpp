@@ -55,8 +55,11 @@ struct SchedState {
 __device__ __forceinline__ float log2p1(double x) { return static_cast<float>(log2(1.0 + x)); }
 
 // phase 0 = batch start (counters of the batch so far are ignored), 1 = round end.
+// active: vertices listed for the round just run (both lists and both queues),
+// as the host read them before the round's pushes.
 __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, SchedState* S, int phase,
-                                                 int k, float inc_frac, int rounds_since, int last_levels) {
+                                                 int k, float inc_frac, int rounds_since, int last_levels,
+                                                 double active) {
   const int lane = threadIdx.x;
   // totals of the work counters (kSlots copies each)
   unsigned long long pu = D.ctrs[C_PUSH * kSlots + lane];
@@ -73,9 +76,6 @@ __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, S
     f[6] = 0.f;
     f[7] = log2p1(last_levels) / 8.f;
   } else {
-    const Ctrl& c = *D.ctrl;
-    const double active = static_cast<double>(c.wl_cnt) + c.wl2_cnt +
-                          static_cast<double>(c.wq[kQActive] >> 32) + static_cast<double>(c.wq[kQPull] >> 32);
     const long long ds = sink - S->prev_sink;
     f[0] = log2p1(active) / ln;
     f[1] = log2p1(static_cast<double>(pu - S->prev_push)) / lm;

@@ -11,9 +11,16 @@
 //   dmf_input updates <edges.txt> <out_prefix> <pct_per_file> <inc_pct> <seed> [lo hi]
 //        <out_prefix>_p1..p10 .txt/.bin, file i = first i*B records of one shuffled
 //        update list, B = pct_per_file% of the input lines (cut_p.cpp:32-38 nesting)
+//   dmf_input tree <edges.txt> <updates_dir> <prefix> <pct_per_file> <seed> [lo hi]
+//        the reference's update tree (input_create/create_txt_updates.sh:17-49,
+//        input_create/README.md:14-21): <updates_dir>/{dec,inc,mixed}/<prefix>_p1..p10
+//        .txt/.bin as `updates` writes them, with 0%, 100% and 50% increases
 //   dmf_input successive <edges.txt> <out_prefix> <inc_pct> <seed> [lo hi]
 //        <out_prefix>_b1..b10 .bin, batch i = i% of the input lines, each drawn from
 //        the capacities left by batches 1..i-1
+#include <sys/stat.h>
+
+#include <cerrno>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -70,6 +77,36 @@ int emit_graph(const std::string& prefix, int64_t k, int32_t* u, int32_t* v, int
   dmf_csr_free(&g);
   return 0;
 }
+
+// Ten nested update files <prefix>_p1..p10 (.txt and .bin): file i holds the
+// first i*B records of one update list of 10*B distinct input edges, B =
+// pct_per_file% of the input lines (cut_p.cpp:32-38 nesting).
+int write_nested(const std::string& prefix, const std::vector<int32_t>& u, const std::vector<int32_t>& v,
+                 const std::vector<int32_t>& w, int pct_per_file, double inc, uint64_t seed, int lo, int hi) {
+  const int64_t k = (int64_t)u.size();
+  std::vector<int32_t> cur = w;
+  const int64_t B = k * pct_per_file / 100;
+  int32_t cnt = 0;
+  int32_t *a = nullptr, *b = nullptr, *c = nullptr;
+  int rc = dmf_gen_updates(k, u.data(), v.data(), cur.data(), 10 * B, inc, lo, hi, seed, 0, &cnt, &a, &b, &c);
+  if (rc) return die(dmf_strerror(rc));
+  for (int i = 1; i <= 10 && !rc; ++i) {
+    const int64_t take = std::min<int64_t>(cnt, (int64_t)i * B);
+    const std::string base = prefix + "_p" + std::to_string(i);
+    if (!write_edges(base + ".txt", a, b, c, take, true)) {
+      rc = die("cannot write updates");
+      break;
+    }
+    const int wr = dmf_write_updates_bin((base + ".bin").c_str(), (int32_t)take, a, b, c);
+    if (wr) rc = die(dmf_strerror(wr));
+  }
+  dmf_free(a);
+  dmf_free(b);
+  dmf_free(c);
+  return rc;
+}
+
+bool make_dir(const std::string& path) { return mkdir(path.c_str(), 0755) == 0 || errno == EEXIST; }
 
 }  // namespace
 
@@ -148,6 +185,24 @@ int main(int argc, char** argv) {
     dmf_free(w);
     return rc;
   }
+  if (cmd == "tree" && argc >= 7) {
+    std::vector<int32_t> u, v, w;
+    if (!read_edges(argv[2], u, v, &w)) return die("cannot read input");
+    const std::string dir = argv[3], prefix = argv[4];
+    const int pct = std::atoi(argv[5]);
+    const uint64_t seed = std::strtoull(argv[6], nullptr, 10);
+    const int lo = argc > 8 ? std::atoi(argv[7]) : 1, hi = argc > 8 ? std::atoi(argv[8]) : 100;
+    if (!make_dir(dir)) return die("cannot create the updates directory");
+    // the three update types of create_txt_updates.sh:17-24
+    const struct { const char* name; double inc; } types[] = {{"dec", 0.0}, {"inc", 1.0}, {"mixed", 0.5}};
+    for (int i = 0; i < 3; ++i) {
+      const std::string sub = dir + "/" + types[i].name;
+      if (!make_dir(sub)) return die("cannot create an update type directory");
+      const int rc = write_nested(sub + "/" + prefix, u, v, w, pct, types[i].inc, seed + (uint64_t)i, lo, hi);
+      if (rc) return rc;
+    }
+    return 0;
+  }
   if ((cmd == "updates" && argc >= 7) || (cmd == "successive" && argc >= 6)) {
     std::vector<int32_t> u, v, w;
     if (!read_edges(argv[2], u, v, &w)) return die("cannot read input");
@@ -160,25 +215,7 @@ int main(int argc, char** argv) {
     const int hi = argc > ai + 1 ? std::atoi(argv[ai + 1]) : 100;
     const int64_t k = (int64_t)u.size();
     std::vector<int32_t> cur = w;
-    if (!succ) {
-      const int64_t B = k * std::atoi(argv[4]) / 100;
-      int32_t cnt = 0;
-      int32_t *a = nullptr, *b = nullptr, *c = nullptr;
-      int rc = dmf_gen_updates(k, u.data(), v.data(), cur.data(), 10 * B, inc, lo, hi, seed, 0, &cnt,
-                               &a, &b, &c);
-      if (rc) return die(dmf_strerror(rc));
-      for (int i = 1; i <= 10; ++i) {
-        const int64_t take = std::min<int64_t>(cnt, (int64_t)i * B);
-        const std::string base = prefix + "_p" + std::to_string(i);
-        if (!write_edges(base + ".txt", a, b, c, take, true)) return die("cannot write updates");
-        rc = dmf_write_updates_bin((base + ".bin").c_str(), (int32_t)take, a, b, c);
-        if (rc) return die(dmf_strerror(rc));
-      }
-      dmf_free(a);
-      dmf_free(b);
-      dmf_free(c);
-      return 0;
-    }
+    if (!succ) return write_nested(prefix, u, v, w, std::atoi(argv[4]), inc, seed, lo, hi);
     for (int i = 1; i <= 10; ++i) {
       int32_t cnt = 0;
       int32_t *a = nullptr, *b = nullptr, *c = nullptr;

@@ -131,3 +131,32 @@ def test_workload_drivers_refuse_without_device():
     r = subprocess.run([sys.executable, "-m", "dyn_maxflow_amd.grid_bench", "--side", "8"], capture_output=True,
                        text=True, cwd=os.path.dirname(os.path.dirname(BIN)))
     assert r.returncode != 0 and "no HIP device" in r.stderr
+
+
+def test_tree_command_writes_the_update_type_layout(tmp_path, golden_dir):
+    """dmf_input tree writes input_create's layout (create_txt_updates.sh:17-49):
+    <dir>/{dec,inc,mixed}/<prefix>_p1..p10, nested prefixes, with only
+    decreases, only increases, and half of each."""
+    g = dmf.read_graph_bin(os.path.join(golden_dir, "johnson.bin"))
+    edges = np.loadtxt(os.path.join(golden_dir, "johnson_w.txt"), dtype=np.int64)
+    cap = {(int(a), int(b)): int(c) for a, b, c in edges}
+    r = tool("tree", os.path.join(golden_dir, "johnson_w.txt"), tmp_path / "updates", "johnson", 5, 7)
+    assert r.returncode == 0, r.stderr
+    for kind in ("dec", "inc", "mixed"):
+        prev = None
+        ups = 0
+        for i in range(1, 11):
+            a, b, c = dmf.read_updates_bin(str(tmp_path / "updates" / kind / f"johnson_p{i}.bin"))
+            assert len(a) == i * (210 * 5 // 100)
+            if prev is not None:
+                assert np.array_equal(a[: len(prev)], prev)
+            prev = a
+            if i == 10:
+                ups = sum(int(w) > cap[(int(x), int(y))] for x, y, w in zip(a, b, c))
+                if kind == "dec":
+                    assert ups == 0
+                elif kind == "inc":
+                    assert ups == len(a)
+                else:
+                    assert 0 < ups < len(a)
+    assert g.n > 0

@@ -88,6 +88,10 @@ struct dmf_engine {
   // start on them instead of relabelling first: its updates can only make
   // them stale, and convergence is accepted on fresh labels only.
   bool labels_fresh = false;
+  // Early-exit relabel (BFS from the sinks stopped once every active vertex is
+  // labelled): 0 never, 1 under the learned scheduler (default), 2 under every
+  // data-driven schedule.  DMF_EARLY_RELABEL overrides.
+  int early_mode = 1;
   int visited_last = 0;
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
   int32_t ninst = 1;              // independent instances stored as one disjoint union
@@ -187,14 +191,14 @@ int bfs_blocks(int64_t cnt, int64_t chunks) {
 }
 
 // Stepped global relabel: host loop over levels.  Returns visited count.
-int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited) {
+int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited, bool early) {
   Dev& D = E->D;
   int rc;
-  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, nullptr);
+  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, nullptr, early ? 1 : 0);
   ++E->launches;
   HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
-  else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode);
+  else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode, early ? 1 : 0);
   int lo = 0, lvl = 0;
   for (;;) {
     if ((rc = sync_ctrl(E))) return rc;
@@ -275,14 +279,14 @@ void destroy_segments(dmf_engine* E) {
 // check per segment instead of one per level.  The first segment is as long as
 // the previous BFS of this mode was deep; later ones grow.  Same result as
 // relabel_stepped (the same level kernel body runs on the same frontiers).
-int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
+int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool early) {
   Dev& D = E->D;
   int rc;
-  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev);
+  hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev, early ? 1 : 0);
   ++E->launches;
   HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
-  else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode);
+  else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode, early ? 1 : 0);
   int idx = 1;
   while (idx + 1 < dmf_engine::kSegSizes && (1 << idx) < E->seg_levels[mode] + 1) ++idx;
   int lvl = 0;
@@ -312,17 +316,19 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited) {
   return DMF_OK;
 }
 
-int relabel(dmf_engine* E, int mode, int part_cnt, int* visited) {
-  const Dev& D = E->D;
+// early (BFS_SINK only): stop once every vertex active now is labelled; the
+// control block's bfs_cut then tells that the labels are not complete.
+int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = false) {
+  early = early && mode == BFS_SINK;
   if (E->use_graph && !E->graph_failed) {
-    const int rc = relabel_graph(E, mode, part_cnt, visited);
+    const int rc = relabel_graph(E, mode, part_cnt, visited, early);
     if (rc != kNoGraph) return rc;
     // capture or instantiation refused: keep the host-driven level loop for
     // this engine; relabel_stepped starts the BFS over from its roots
     E->graph_failed = true;
     std::fprintf(stderr, "dmf: BFS graph capture unavailable, using per-level launches\n");
   }
-  return relabel_stepped(E, mode, part_cnt, visited);
+  return relabel_stepped(E, mode, part_cnt, visited, early);
 }
 
 int reset_lists(dmf_engine* E) {
@@ -421,13 +427,21 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         want_relabel = false;
       }
     }
+    // Early-exit relabels label only as deep as the active vertices lie, so
+    // they never count as fresh: the first relabel of a solve and every
+    // relabel that confirms convergence are full.
+    const bool try_early = !topo && E->D.tgt && (learned ? E->early_mode >= 1 : E->early_mode >= 2);
+    bool need_full = true;
     for (;;) {
       bool fresh = false;
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
-        if ((rc = relabel(E, BFS_SINK, 0, &visited))) return rc;
+        const bool early = try_early && !need_full;
+        if ((rc = relabel(E, BFS_SINK, 0, &visited, early))) return rc;
         since = 0;
-        fresh = true;
+        fresh = !(early && E->h_ctrl->bfs_cut);
+        if (!fresh) LAUNCH(k_cap_unlabelled, blocks_for(n), D, std::max(0, E->last_levels - 1));
+        need_full = false;
       }
       int32_t cnt = 0;
       unsigned long long wpk = 0;
@@ -441,6 +455,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
           if (fresh) break;
           since = C.interval;  // confirm convergence on fresh labels
           want_relabel = true;
+          need_full = true;
           continue;
         }
       }
@@ -1000,6 +1015,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   Ctrl* ctrl = nullptr;
   unsigned long long* hkey = nullptr;
   uint16_t* lev16 = nullptr;
+  uint8_t* tgt = nullptr;
   int rc = DMF_OK;
   const bool alloc_failed = (rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
       (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
@@ -1010,7 +1026,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
-      (rc = dalloc(E, &lev16, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
+      (rc = dalloc(E, &lev16, n)) || (rc = dalloc(E, &tgt, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
       (!E->h_vmap.empty() && (rc = dalloc(E, &E->d_vmap, n)));
   D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
   D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
@@ -1018,6 +1034,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
   D.lev16 = lev16;
+  D.tgt = tgt;
   D.vmap = E->d_vmap;
   if (alloc_failed) {
     dmf_engine_destroy(E);
@@ -1029,6 +1046,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
     // unreachable and a bottom-up level scans all of it)
     const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
     D.bottom_up = bu ? std::atoi(bu) : 0;
+    if (const char* ee = std::getenv("DMF_EARLY_RELABEL")) E->early_mode = std::atoi(ee);
   }
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
     dmf_engine_destroy(E);
@@ -1331,6 +1349,7 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(const_cast<int32_t*>(D.inst));
     hipFree(D.iflow);
     hipFree(D.lev16);
+    hipFree(D.tgt);
     hipFree(const_cast<uint32_t*>(D.widebits));
     hipFree(D.wq_vert);
     hipFree(D.wq_base);
@@ -1564,6 +1583,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += 7 * 4 * M;                               // col rev cap cap0 res rres owner
   b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst or vmap
   b += 2 * N;                                   // 16-bit BFS levels
+  b += N;                                       // early-exit relabel targets
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)

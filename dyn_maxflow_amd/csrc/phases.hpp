@@ -55,6 +55,8 @@ enum CtrIdx {
   C_ATOMIC,
   C_BFSARC,   // arc visits of BFS level launches (also counted in C_ARC)
   C_BFSVERT,  // vertex visits of BFS level launches (also counted in C_VERT)
+  C_TGT,      // early-exit relabel: targets marked by the BFS init
+  C_TGTF,     // early-exit relabel: targets labelled so far
   C_N = 16
 };
 
@@ -76,6 +78,8 @@ struct Ctrl {
   int32_t visited;    // vertices labelled by the last global relabel
   int32_t rounds;     // rounds of the last solve
   int32_t inc;        // updates that raise an arc's capacity (scheduler feature)
+  int32_t early;      // the current relabel may stop once its targets are labelled
+  int32_t bfs_cut;    // the current relabel stopped before its frontier ran out
   int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
                       // read from slot r and written to slot r^1 by each level
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
@@ -134,6 +138,10 @@ struct Dev {
   // Engine numbering of each caller vertex id, or null when the engine keeps
   // the caller's numbering.  Update records carry caller ids.
   const int32_t* vmap;
+  // Early-exit relabel: tgt[v] = 1 for the vertices active when the relabel
+  // started (the BFS may stop once all of them are labelled); null when the
+  // engine has no such relabel.
+  uint8_t* tgt;
   // current update batch
   const int32_t* usrc;
   const int32_t* udst;
@@ -592,13 +600,22 @@ __device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1]
 // / pp_bfs_init (code/dyn_pp.h:76-83): every vertex unlabelled (height n),
 // roots labelled 0 and appended to order[] (wide roots also to wide queue 0).
 // ctrl->qcnt[0] and ctrl->wq[0] must be 0.
-__device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
+// early (BFS_SINK only): also mark as targets the vertices that are active
+// now (not a terminal, excess > 0, height < n before this relabel) and count
+// them.
+__device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t, bool early = false) {
   const int n = D.n;
+  long long targets = 0;
   for (int base = r.gid - lane_id(); base < n; base += r.gsz) {
     const int v = base + lane_id();
     bool root = false;
     if (v < n) {
       const long long ex = D.excess[v];
+      if (early) {
+        const bool tg = !is_term(D, v) && ex > 0 && D.height[v] < n;
+        D.tgt[v] = tg ? 1 : 0;
+        targets += tg ? 1 : 0;
+      }
       if (mode == BFS_SINK) {
         root = is_snk(D, v) || (!is_src(D, v) && ex < 0);
       } else if (mode == BFS_CT) {
@@ -621,6 +638,30 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t) {
     const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
     if (root) D.order[off + rank] = v;
   }
+  if (early) {
+    targets = wave_sum_i64(targets);
+    if (lane_id() == 0 && targets) atomicAdd(&D.ctrs[C_TGT * kSlots + (r.wid % kSlots)], targets);
+  }
+}
+
+// Whether every target of the current early-exit relabel is labelled.  All 64
+// lanes; the counts of earlier launches are complete at a launch boundary.
+__device__ __forceinline__ bool targets_labelled(const Dev& D) {
+  const long long a = static_cast<long long>(D.ctrs[C_TGT * kSlots + lane_id()]);
+  const long long b = static_cast<long long>(D.ctrs[C_TGTF * kSlots + lane_id()]);
+  return wave_sum_i64(b) >= wave_sum_i64(a);
+}
+// The same decision taken once for the whole block (wave 0 reads the counts),
+// so that every wave of a block returns or expands together.  Every thread of
+// the block must call it.
+__device__ __forceinline__ bool block_targets_labelled(const Dev& D) {
+  __shared__ int done;
+  if (threadIdx.x < kWave) {
+    const bool d = targets_labelled(D);
+    if (threadIdx.x == 0) done = d ? 1 : 0;
+  }
+  __syncthreads();
+  return done != 0;
 }
 
 // final_bfs_init (code/dyn_pp.h:86-93): roots are the part's deficits.
@@ -704,8 +745,9 @@ struct BfsTiles {
 // residuals are loaded before the current step's labels are tested, and all
 // of a step's claims are issued before any of their results is waited for,
 // so each wave keeps several dependent round trips' worth of arcs in flight.
+// early: count the targets this level labels (early-exit relabel).
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
-                             int32_t* next_cnt, const WList& W, int qnext, Tally& t) {
+                             int32_t* next_cnt, const WList& W, int qnext, Tally& t, bool early = false) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
   __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
   Stage st{sbuf[threadIdx.x / kWave], 0};
@@ -715,6 +757,7 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
   const int nb = lvl + 1;
   const uint16_t nb16 = static_cast<uint16_t>(nb < kLevFree ? nb : kLevFree - 1);
   int32_t* out = D.order + lo + cnt;
+  long long found = 0;
   auto expand = [&](const BfsTiles& T) {
     bool c[kBfsTiles];
 #pragma unroll
@@ -732,6 +775,7 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
     for (int j = 0; j < kBfsTiles; ++j) {
       const bool hit = old[j] == D.n;
       if (hit) D.lev16[T.w[j]] = nb16;
+      if (hit && early) found += D.tgt[T.w[j]];
       wide_note(D, WS, qnext, hit && is_wide(D, T.w[j]), T.w[j]);
       st.push(hit, T.w[j], next_cnt, out);
     }
@@ -801,6 +845,10 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
       if (k0 + kStep < total) cur = fetch(k0 + kStep);
       expand(now);
     }
+  }
+  if (early) {
+    found = wave_sum_i64(found);
+    if (lane_id() == 0 && found) atomicAdd(&D.ctrs[C_TGTF * kSlots + (r.wid % kSlots)], found);
   }
   st.flush_block(next_cnt, out);  // its barriers also publish the wide stage
   wide_flush(D, WS, qnext);
@@ -1718,14 +1766,33 @@ __global__ void __launch_bounds__(kBlock) k_begin(Dev D) {
 // k_relabel_prep: the frontier ring, the BFS wide queues and the replay
 // cursor, before a global relabel; with out set, also the device copy of Dev
 // that graph-replayed levels read.
-__global__ void k_relabel_prep(Dev D, Dev* out) {
+// early: the relabel may stop once every target is labelled (its target
+// counters are zeroed here).  Launched with one wave.
+__global__ void k_relabel_prep(Dev D, Dev* out, int early) {
   if (threadIdx.x == 0) {
     Ctrl* c = D.ctrl;
     c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
     c->wq[0] = c->wq[1] = c->wq[2] = 0;
     c->bcur[0][0] = c->bcur[0][1] = c->bcur[1][0] = c->bcur[1][1] = 0;
+    c->early = early;
+    c->bfs_cut = 0;
     if (out) *out = D;
   }
+  if (threadIdx.x < kSlots) {
+    D.ctrs[C_TGT * kSlots + threadIdx.x] = 0;
+    D.ctrs[C_TGTF * kSlots + threadIdx.x] = 0;
+  }
+}
+// After an early-exit relabel whose deepest label is cap: vertices it left
+// unlabelled and that hold no excess get height cap, so that the labels stay
+// valid (h(w) <= h(v) + 1 for every residual arc w->v; unlabelled vertices lie
+// at least one level beyond the last level that was expanded in full) and
+// the label repair does not take them for violations.  Unlabelled vertices
+// with excess keep height n until a full relabel decides whether they can
+// still reach a sink.
+__global__ void __launch_bounds__(kBlock) k_cap_unlabelled(Dev D, int cap) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += gridDim.x * blockDim.x)
+    if (D.height[v] == D.n && D.excess[v] <= 0) D.height[v] = cap;
 }
 // k_flow_prep: list counters, flags, the flow and the per-instance flows,
 // before the flow reduction.
@@ -1751,9 +1818,9 @@ __global__ void __launch_bounds__(kBlock) k_saturate_sink_pp(Dev D) {
   ph_saturate_sink_pp(D, make_range(), t);
   t.flush(D.ctrs);
 }
-__global__ void __launch_bounds__(kBlock) k_bfs_init(Dev D, int mode) {
+__global__ void __launch_bounds__(kBlock) k_bfs_init(Dev D, int mode, int early) {
   Tally t;
-  ph_bfs_init(D, make_range(), mode, t);
+  ph_bfs_init(D, make_range(), mode, t, early != 0);
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_bfs_init_final(Dev D, int cnt) {
@@ -1772,11 +1839,18 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
     D.ctrl->qcnt[(lvl + 2) % 3] = 0;
     D.ctrl->wq[(lvl + 2) % 3] = 0;
   }
+  // early-exit relabel whose targets are all labelled: the level is not
+  // expanded, nothing is appended, and the host sees an empty next level
+  const bool early = D.ctrl->early != 0;
+  if (early && block_targets_labelled(D)) {
+    if (threadIdx.x == 0) D.ctrl->bfs_cut = 1;
+    return;
+  }
   if (bottomup) {
     ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
   } else {
     const WList W = wlist(D, lvl % 3, wpk);
-    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+    ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, early);
   }
   t.flush(D.ctrs, true);
 }
@@ -1812,12 +1886,21 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
     }
   }
   if (cnt == 0) return;  // uniform: the BFS is over
+  // early-exit relabel whose targets are all labelled: this level is not
+  // expanded, so the next one comes out empty.  A block that starts late may
+  // see counts that blocks still running have raised and stop while others
+  // expand; a partly expanded level still labels exact distances.
+  const bool early = cc->early != 0;
+  if (early && block_targets_labelled(D)) {
+    if (threadIdx.x == 0) D.ctrl->bfs_cut = 1;
+    return;
+  }
   Tally t;
   if (use_bottom_up(D, mode, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
     ph_bfs_bottomup(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
   } else {
     const WList W = wlist(D, lvl % 3, wpk);
-    ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t);
+    ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, early);
   }
   t.flush(D.ctrs, true);
 }

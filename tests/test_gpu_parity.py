@@ -505,3 +505,25 @@ def test_lazy_and_learned_batches_without_min_cut(gpu):
                 check_engine(eng, g, caps_after(g, bl), s, t, f, f"{sched} last batch")
             runs[(sched, between)] = ctr
         assert runs[(sched, False)] == runs[(sched, True)], sched
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("interval", [1, 3])
+def test_early_exit_relabel_keeps_results(gpu, mode, interval, monkeypatch):
+    """Relabels that stop once every active vertex is labelled (forced on for
+    the fixed schedule here; the learned scheduler uses them by default) end
+    exact: convergence is only accepted after a full relabel."""
+    dmf = gpu
+    monkeypatch.setenv("DMF_EARLY_RELABEL", "2")
+    g, s, t, (u, v, w) = rmat_case(12, 16, 13)
+    p = dmf.default_params(g, relabel_interval=interval, mode=mode)
+    say(f"[early] mode={mode} interval={interval}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(dmf.default_params(g, variant="pp", mode=mode))
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b, pct in enumerate((1, 4, 2, 8)):
+            applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1300 + b))
+            f = eng.apply_batch(*applied[-1], p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"early batch {b}")

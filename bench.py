@@ -379,8 +379,11 @@ def main():
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels of the plain passes")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")
-    ap.add_argument("--variants", default="data,pp,topo",
-                    help="variants of the plain fixed-schedule passes (the first is the headline's plain)")
+    ap.add_argument("--variants", default="data,pp,topo,data+cascade",
+                    help="variants of the plain fixed-schedule passes (the first is the headline's plain; "
+                         "+cascade adds the engine's cascade sub-rounds)")
+    ap.add_argument("--cascade", type=int, default=1024,
+                    help="cascade sub-round cap of the +cascade passes")
     ap.add_argument("--static-variant", default="pp",
                     help="variant of the untimed static solve that every repeat starts from (the same for "
                          "every pass; push-pull lets every variant take the batches)")
@@ -435,7 +438,14 @@ def main():
     sides = {}
     variants, flows = {}, None
     for vi, var in enumerate(names):
-        recs, fl = run_gpu(g, s, t, batches, plain_params(var), args.steps, args.warmup, f"plain {var}",
+        # "data+cascade": the reference's fixed schedule on the engine's cascade
+        # sub-rounds and early-exit relabels (the mechanisms the learned
+        # scheduler runs on), to separate them from the scheduler's decisions
+        base = var.split("+")[0]
+        pp = plain_params(base)
+        if var.endswith("+cascade"):
+            pp.cascade = args.cascade
+        recs, fl = run_gpu(g, s, t, batches, pp, args.steps, args.warmup, f"plain {var}",
                            static_params=static, sides=sides if vi == 0 else None)
         variants[var] = summarize(recs)
         if flows is None:

@@ -63,7 +63,7 @@ class _Csr(C.Structure):
 
 
 class Params(C.Structure):
-    """dmf_params: variant, kc, relabel_interval, scheduler, max_rounds, mode, lazy_start."""
+    """dmf_params: variant, kc, relabel_interval, scheduler, max_rounds, mode, lazy_start, cascade."""
 
     _fields_ = [
         ("variant", C.c_int32),
@@ -73,6 +73,7 @@ class Params(C.Structure):
         ("max_rounds", C.c_int32),
         ("mode", C.c_int32),
         ("lazy_start", C.c_int32),
+        ("cascade", C.c_int32),
     ]
 
 
@@ -414,7 +415,7 @@ def device_count() -> int:
 
 
 def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=None,
-                   max_rounds=0, scheduler=SCHED_FIXED, lazy_start=0) -> Params:
+                   max_rounds=0, scheduler=SCHED_FIXED, lazy_start=0, cascade=0) -> Params:
     """dmf_params_default, then the given fields (mode None keeps the engine's
     default driver: stepped with graph-replayed BFS levels)."""
     p = Params()
@@ -429,6 +430,7 @@ def default_params(g: Graph, variant="data", kc=None, relabel_interval=1, mode=N
     p.max_rounds = max_rounds
     p.scheduler = scheduler
     p.lazy_start = lazy_start
+    p.cascade = cascade
     return p
 
 

@@ -89,9 +89,18 @@ struct dmf_engine {
   // them stale, and convergence is accepted on fresh labels only.
   bool labels_fresh = false;
   // Early-exit relabel (BFS from the sinks stopped once every active vertex is
-  // labelled): 0 never, 1 under the learned scheduler (default), 2 under every
+  // labelled): 0 never, 1 with cascade sub-rounds (default), 2 under every
   // data-driven schedule.  DMF_EARLY_RELABEL overrides.
   int early_mode = 1;
+  // Cascade push sub-rounds after each data-driven round's push and repair:
+  // DMF_CASCADE overrides the solve's cap when set (>= 0); a sub-round gives
+  // each listed vertex cascade_kc discharge cycles and lists only the
+  // vertices it gave excess (cascade_relist 0; DMF_CASCADE_KC and
+  // DMF_CASCADE_RELIST override both).  Stamps of the lists only grow.
+  int cascade_env = -1;
+  int cascade_kc = 1;
+  int cascade_relist = 0;
+  int32_t cstamp = 0;
   int visited_last = 0;
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
   int32_t ninst = 1;              // independent instances stored as one disjoint union
@@ -142,6 +151,10 @@ int dalloc(dmf_engine* E, T** p, size_t count) {
 }
 
 constexpr int kGridBlocksPerCu = 4;
+// Cascade sub-rounds launched between host checks, and the learned
+// scheduler's cap on them per round when the parameters give none.
+constexpr int kCascadeChunk = 8;
+constexpr int kLearnedCascade = 1024;
 
 int blocks_for(int64_t work) {
   int64_t b = (work + kBlock - 1) / kBlock;
@@ -430,7 +443,13 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     // Early-exit relabels label only as deep as the active vertices lie, so
     // they never count as fresh: the first relabel of a solve and every
     // relabel that confirms convergence are full.
-    const bool try_early = !topo && E->D.tgt && (learned ? E->early_mode >= 1 : E->early_mode >= 2);
+    // Cascade sub-rounds: the fixed schedules only when asked for, the learned
+    // scheduler by default; the chain is launched in chunks and stops when a
+    // chunk leaves no vertex listed.
+    int sub = topo ? 0 : (C.cascade > 0 ? C.cascade : (learned ? kLearnedCascade : 0));
+    if (E->cascade_env >= 0) sub = topo ? 0 : E->cascade_env;
+    if (!E->D.cmark) sub = 0;
+    const bool try_early = !topo && E->D.tgt && (E->early_mode >= 2 || (E->early_mode == 1 && sub > 0));
     bool need_full = true;
     for (;;) {
       bool fresh = false;
@@ -471,9 +490,29 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         if (!E->h_ctrl->flag[fs]) break;
         active = static_cast<double>(n);
       } else {
-        LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0);
+        LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
+               sub ? ++E->cstamp : 0, E->cascade_relist);
         LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
         active = static_cast<double>(cnt) + static_cast<double>(wpk >> 32);
+        // cascade sub-rounds: each pushes from the list the pass before it
+        // wrote, launched kCascadeChunk at a time with one host check between
+        // chunks (an empty list ends the chain)
+        if (sub > 0) {
+          const int cgrid = std::max(1, std::min(E->grid_cap, 2 * std::max(1, E->num_cu)));
+          const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
+          int slot = 0;
+          for (int done_sub = 0; done_sub < sub;) {
+            const int chunk = std::min(kCascadeChunk, sub - done_sub);
+            for (int j = 0; j < chunk; ++j) {
+              LAUNCH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist);
+              slot = (slot + 1) % 3;
+            }
+            done_sub += chunk;
+            if (done_sub >= sub) break;
+            if ((rc = sync_ctrl(E))) return rc;
+            if (E->h_ctrl->qcnt[slot] == 0 && E->h_ctrl->wq[slot] == 0) break;
+          }
+        }
       }
       ++round;
       ++since;
@@ -593,6 +632,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
     d.max_rounds = 0;
     d.mode = 0;
     d.lazy_start = 0;
+    d.cascade = 0;
     p = &d;
   }
   c.variant = p->variant;
@@ -604,6 +644,7 @@ Cfg make_cfg(const dmf_engine* E, const dmf_params* p, int entry) {
   c.entry = entry;
   c.scheduler = p->scheduler;
   c.lazy = p->lazy_start;
+  c.cascade = p->cascade > 0 ? p->cascade : 0;
   return c;
 }
 
@@ -681,10 +722,10 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   if (E->trace) {
     std::fprintf(E->trace,
                  "{\"solve\":%llu,\"what\":\"summary\",\"entry\":%d,\"variant\":%d,\"scheduler\":%d,"
-                 "\"interval\":%d,\"lazy\":%d,\"k\":%d,\"device_ms\":%.4f,\"wall_ms\":%.4f,\"rounds\":%llu,"
-                 "\"relabels\":%llu,\"rc\":%d}\n",
-                 (unsigned long long)E->solve_id, entry, E->last_variant, C.scheduler, C.interval, C.lazy, E->D.k,
-                 E->last.device_ms, since_solve_ms(E), (unsigned long long)E->last.rounds,
+                 "\"interval\":%d,\"lazy\":%d,\"cascade\":%d,\"k\":%d,\"device_ms\":%.4f,\"wall_ms\":%.4f,"
+                 "\"rounds\":%llu,\"relabels\":%llu,\"rc\":%d}\n",
+                 (unsigned long long)E->solve_id, entry, E->last_variant, C.scheduler, C.interval, C.lazy, C.cascade,
+                 E->D.k, E->last.device_ms, since_solve_ms(E), (unsigned long long)E->last.rounds,
                  (unsigned long long)E->last.global_relabels, rc);
     std::fflush(E->trace);
   }
@@ -1016,6 +1057,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   unsigned long long* hkey = nullptr;
   uint16_t* lev16 = nullptr;
   uint8_t* tgt = nullptr;
+  int32_t* cmark = nullptr;
   int rc = DMF_OK;
   const bool alloc_failed = (rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
       (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
@@ -1026,7 +1068,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
-      (rc = dalloc(E, &lev16, n)) || (rc = dalloc(E, &tgt, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
+      (rc = dalloc(E, &lev16, n)) || (rc = dalloc(E, &tgt, n)) || (rc = dalloc(E, &cmark, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
       (!E->h_vmap.empty() && (rc = dalloc(E, &E->d_vmap, n)));
   D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
   D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
@@ -1035,6 +1077,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
   D.lev16 = lev16;
   D.tgt = tgt;
+  D.cmark = cmark;
   D.vmap = E->d_vmap;
   if (alloc_failed) {
     dmf_engine_destroy(E);
@@ -1047,6 +1090,9 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
     const char* bu = std::getenv("DMF_BFS_BOTTOM_UP");
     D.bottom_up = bu ? std::atoi(bu) : 0;
     if (const char* ee = std::getenv("DMF_EARLY_RELABEL")) E->early_mode = std::atoi(ee);
+    if (const char* cs = std::getenv("DMF_CASCADE")) E->cascade_env = std::max(0, std::atoi(cs));
+    if (const char* ck = std::getenv("DMF_CASCADE_KC")) E->cascade_kc = std::max(0, std::atoi(ck));
+    if (const char* cr = std::getenv("DMF_CASCADE_RELIST")) E->cascade_relist = std::atoi(cr);
   }
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
     dmf_engine_destroy(E);
@@ -1074,6 +1120,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
             hipMemsetAsync(excess, 0, sizeof(long long) * n, E->stream) == hipSuccess &&
             hipMemsetAsync(height, 0, sizeof(int32_t) * n, E->stream) == hipSuccess &&
             hipMemsetAsync(side, 0, n, E->stream) == hipSuccess &&
+            hipMemsetAsync(cmark, 0, sizeof(int32_t) * n, E->stream) == hipSuccess &&
             hipMemsetAsync(pfinal, 0, n, E->stream) == hipSuccess &&
             hipMemsetAsync(ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess &&
             hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
@@ -1263,6 +1310,7 @@ void dmf_params_default(const dmf_csr* g, dmf_params* p) {
   p->max_rounds = 0;
   p->mode = 2;  // stepped with graph-replayed BFS levels: the fastest driver
   p->lazy_start = 0;
+  p->cascade = 0;
 }
 
 int dmf_engine_create(const dmf_csr* g, int32_t s, int32_t t, dmf_engine** out) {
@@ -1350,6 +1398,7 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(D.iflow);
     hipFree(D.lev16);
     hipFree(D.tgt);
+    hipFree(D.cmark);
     hipFree(const_cast<uint32_t*>(D.widebits));
     hipFree(D.wq_vert);
     hipFree(D.wq_base);
@@ -1584,6 +1633,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += 8 * N + 5 * 4 * N + 3 * N + 4 * N;       // excess; height order wl wl2 part; side pfinal term; inst or vmap
   b += 2 * N;                                   // 16-bit BFS levels
   b += N;                                       // early-exit relabel targets
+  b += 4 * N;                                   // cascade list stamps
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)

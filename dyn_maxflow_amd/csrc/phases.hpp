@@ -142,6 +142,9 @@ struct Dev {
   // started (the BFS may stop once all of them are labelled); null when the
   // engine has no such relabel.
   uint8_t* tgt;
+  // Cascade push sub-rounds: cmark[v] is the stamp of the last sub-round that
+  // listed v (stamps only grow, so no clearing); null when not allocated.
+  int32_t* cmark;
   // current update batch
   const int32_t* usrc;
   const int32_t* udst;
@@ -159,6 +162,7 @@ struct Cfg {
   int32_t entry;  // 0 = static (init + saturate), 1 = batch (apply + repair)
   int32_t scheduler;  // DMF_SCHED_*
   int32_t lazy;       // batch may start on the labels the previous solve ended with
+  int32_t cascade;    // most cascade push sub-rounds per data-driven round (0 = none)
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;
@@ -1083,6 +1087,43 @@ __device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
   return true;
 }
 
+// Cascade push: after a round's push, sub-rounds on the device push again
+// from the vertices that gained excess or kept some, without a host round
+// trip, a worklist scan or a relabel in between.  Their lists form a ring of
+// three slots: slot s is the narrow list cascade_list(D, s) with size
+// ctrl->qcnt[s] plus the chunked wide queue s (the BFS ring, idle between
+// relabels).  Cascade{slot, stamp}: the slot the current pass appends to and
+// its stamp (slot < 0: no cascade).
+// relist: 0 = a vertex that keeps excess is not listed again (it waits for
+// the next round), 1 = only if it pushed in this pass, 2 = always.
+struct Cascade {
+  int slot;
+  int stamp;
+  int relist;
+};
+__device__ __forceinline__ int32_t* cascade_list(const Dev& D, int slot) {
+  return slot == 0 ? D.wl2 : slot == 1 ? D.part : D.wl;
+}
+// Lanes with cand set note vertex x for the next sub-round, once per stamp.
+// Works on whatever lanes are active: the narrow appends of those lanes take
+// one atomic, the wide ones reserve their chunks in the slot's queue.
+__device__ __forceinline__ void cascade_note(const Dev& D, const Cascade& cz, bool cand, int x) {
+  if (cand) cand = atomicMax(&D.cmark[x], cz.stamp) < cz.stamp;
+  int deg = 0;
+  if (cand) deg = degree(D, x);
+  const bool wide = cand && deg > D.lane_max;
+  const bool narrow = cand && !wide;
+  const unsigned long long b = __ballot(narrow);
+  if (b) {
+    const int leader = __ffsll(static_cast<long long>(b)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(&D.ctrl->qcnt[cz.slot], __popcll(b));
+    base = __shfl(base, leader);
+    if (narrow) cascade_list(D, cz.slot)[base + __popcll(b & ((1ull << lane_id()) - 1ull))] = x;
+  }
+  if (wide) wq_append(D, cz.slot, x, deg);
+}
+
 // Lane path (degree <= lane_max): up to kc discharge steps.  A step scans the
 // arcs once and pushes along every residual arc to a lower neighbour (the arcs
 // the reference's step of code/dyn_data.h:94-129 may use, one per scan there)
@@ -1091,8 +1132,10 @@ __device__ __forceinline__ bool push_filter(const Dev& D, int pmode, int w) {
 // and the residuals of v's arcs, so the excess read at the start bounds the
 // step's pushes and is debited with one atomic at the end.
 template <bool PULL>
-__device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t) {
+__device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t,
+                                 const Cascade& cz = Cascade{-1, 0, 0}) {
   const int n = D.n;
+  bool pushed = false;
   for (int c = 0; c < kc; ++c) {
     const long long ex = ldr64(&D.excess[v]);
     const int hv = ldr(&D.height[v]);
@@ -1139,7 +1182,9 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
           atomicAdd(&D.rres[e], d);
           atomicAdd(&D.res[p], d);
           atomicSub(&D.rres[p], d);
-          add64(&D.excess[w], d);
+          const long long old = add64(&D.excess[w], d);
+          // w turned active: the next cascade sub-round takes it
+          if (cz.slot >= 0) cascade_note(D, cz, old <= 0 && old + d > 0 && !is_term(D, w), w);
         }
         moved += d;
         t.push += 1;
@@ -1150,11 +1195,16 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
     if (moved) {
       add64(&D.excess[v], PULL ? moved : -moved);
       t.atom += 1;
+      pushed = true;
     }
     if (moved == budget) continue;  // spent; re-check for refills
     if (mo == INT_MAX) break;       // no residual arc left
     str(&D.height[v], mo + 1);
     t.relabel += 1;
+  }
+  if (!PULL && cz.slot >= 0) {  // excess left after the cycles: the next sub-round tries again
+    const bool again = cz.relist == 2 || (cz.relist == 1 && pushed);
+    cascade_note(D, cz, again && ldr64(&D.excess[v]) > 0 && ldr(&D.height[v]) < D.n, v);
   }
 }
 
@@ -1167,7 +1217,7 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
 // receives the lowest height among residual arcs that were not lower.
 template <bool PULL>
 __device__ bool wave_discharge(const Dev& D, int v, int hv, int lo, int hi, int pmode,
-                               int* min_other, Tally& t) {
+                               int* min_other, Tally& t, const Cascade& cz = Cascade{-1, 0, 0}) {
   int mo = INT_MAX;
   bool out = false;
   for (int j0 = lo; j0 < hi; j0 += kWave) {
@@ -1216,7 +1266,8 @@ __device__ bool wave_discharge(const Dev& D, int v, int hv, int lo, int hi, int 
         atomicAdd(&D.rres[e], di);
         atomicAdd(&D.res[p], di);
         atomicSub(&D.rres[p], di);
-        add64(&D.excess[w], d);
+        const long long old = add64(&D.excess[w], d);
+        if (cz.slot >= 0) cascade_note(D, cz, old <= 0 && old + d > 0 && !is_term(D, w), w);
       }
       t.push += 1;
       t.atom += 5;
@@ -1234,7 +1285,8 @@ __device__ bool wave_discharge(const Dev& D, int v, int hv, int lo, int hi, int 
 // adjacency, relabelling to (lowest residual neighbour)+1 when a discharge
 // leaves excess behind — the reference's relabel rule.
 template <bool PULL>
-__device__ void wave_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t) {
+__device__ void wave_push_cycles(const Dev& D, int v, int b, int deg, int kc, int pmode, Tally& t,
+                                 const Cascade& cz = Cascade{-1, 0, 0}) {
   const int n = D.n;
   for (int c = 0; c < kc; ++c) {
     long long ex = 0;
@@ -1248,7 +1300,7 @@ __device__ void wave_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
     const long long amt = PULL ? -ex : ex;
     if (!(amt > 0 && (PULL || hv < n))) break;
     int mo;
-    const bool spent = wave_discharge<PULL>(D, v, hv, b, b + deg, pmode, &mo, t);
+    const bool spent = wave_discharge<PULL>(D, v, hv, b, b + deg, pmode, &mo, t, cz);
     if (spent) continue;  // re-check: pushes from neighbours may have refilled it
     if (mo == INT_MAX) break;  // no residual arc at all
     if (lane_id() == 0) {
@@ -1268,7 +1320,15 @@ __device__ void wave_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
 // Wide vertices come through W (one discharge pass over each chunk).
 template <bool PULL>
 __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int cnt, int kc, int pmode,
-                        int32_t* flag, const WList& W, Tally& t) {
+                        int32_t* flag, const WList& W, Tally& t, const Cascade& cz = Cascade{-1, 0, 0}) {
+  // a wide vertex that keeps excess after its wave's work is listed again
+  // (lane 0 reads it; every lane of the wave calls cascade_note)
+  auto note_self = [&](int v) {
+    if (PULL || cz.slot < 0 || cz.relist == 0) return;
+    bool c = false;
+    if (lane_id() == 0) c = ldr64(&D.excess[v]) > 0 && ldr(&D.height[v]) < D.n;
+    cascade_note(D, cz, c, v);
+  };
   const int n = D.n;
   // Wide chunks: the excess read here only filters out inactive vertices (the
   // claim in wave_discharge is the exact test) and a wide vertex's height does
@@ -1294,7 +1354,8 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
         if (is_term(D, v) || (PULL ? -ex0 : ex0) <= 0 || (!PULL && h0 >= n)) continue;
         if (lane_id() == 0 && *flag == 0) *flag = 1;
       }
-      wave_push_cycles<PULL>(D, v, a, b - a, kc, pmode, t);
+      wave_push_cycles<PULL>(D, v, a, b - a, kc, pmode, t, cz);
+      note_self(v);
       continue;
     }
     long long ex = 0;
@@ -1311,7 +1372,8 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
     if (!act) continue;
     if (!list && lane_id() == 0 && *flag == 0) *flag = 1;
     int mo;
-    wave_discharge<PULL>(D, v, hv, a, b, pmode, &mo, t);
+    wave_discharge<PULL>(D, v, hv, a, b, pmode, &mo, t, cz);
+    note_self(v);
   }
   for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
     const int i = base + lane_id();
@@ -1327,7 +1389,7 @@ __device__ void ph_push(const Dev& D, const Range& r, const int32_t* list, int c
       }
       if (v >= 0) t.vert += 1;
     }
-    if (v >= 0) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t);
+    if (v >= 0) lane_push_cycles<PULL>(D, v, b, deg, kc, pmode, t, cz);
   }
 }
 
@@ -1904,7 +1966,14 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
   }
   t.flush(D.ctrs, true);
 }
+// Block 0 also clears the cascade ring (the BFS ring, which this round's
+// relabel, if any, has finished with): the round's push lists into slot 0.
 __global__ void __launch_bounds__(kBlock) k_build_wl(Dev D, int cnt) {
+  if (is_block0_thread0()) {
+    Ctrl* c = D.ctrl;
+    c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+    c->wq[0] = c->wq[1] = c->wq[2] = 0;
+  }
   Tally t;
   ph_build_wl(D, make_range(), cnt, t);
   t.flush(D.ctrs);
@@ -1924,8 +1993,11 @@ __global__ void __launch_bounds__(kBlock) k_build_final_wl(Dev D, int cnt) {
 // arguments, so block 0 clears the list counters for the next round's builder.
 // Topology-driven rounds alternate flag slots: this round reports activity in
 // flag[fslot] and clears flag[fslot ^ 1], which the host read a round ago.
+// cslot >= 0 (which == 1 only): also list, in cascade slot cslot with stamp
+// cstamp, the vertices the first cascade sub-round takes.
 __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int kc, int pmode,
-                                                 unsigned long long wpk, int fslot) {
+                                                 unsigned long long wpk, int fslot, int cslot = -1,
+                                                 int cstamp = 0, int crelist = 0) {
   Tally t;
   const Range r = make_range();
   if (is_block0_thread0()) {
@@ -1940,10 +2012,30 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
   if (which == 0) {
     ph_push<false>(D, r, nullptr, cnt, kc, pmode, &D.ctrl->flag[fslot], wlist_waves(D), t);
   } else if (which == 1) {
-    ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t);
+    ph_push<false>(D, r, D.wl, cnt, kc, pmode, nullptr, wlist(D, kQActive, wpk), t, Cascade{cslot, cstamp, crelist});
   } else {
     ph_push<true>(D, r, D.wl2, cnt, kc, pmode, nullptr, wlist(D, kQPull, wpk), t);
   }
+  t.flush(D.ctrs);
+}
+// One cascade sub-round: pushes from the list in slot `slot` (sizes read from
+// the control block, written by the previous pass) and lists the vertices for
+// the next sub-round in slot (slot+1)%3 with stamp `stamp`.  Block 0 clears
+// slot (slot+2)%3, the input of the sub-round before, for the sub-round
+// after.  An empty list returns at once, so a chain of launches stops doing
+// work when the excess has settled.
+__global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist) {
+  const Ctrl* cc = D.ctrl;
+  const int cnt = cc->qcnt[slot];
+  const unsigned long long wpk = cc->wq[slot];
+  if (is_block0_thread0()) {
+    D.ctrl->qcnt[(slot + 2) % 3] = 0;
+    D.ctrl->wq[(slot + 2) % 3] = 0;
+  }
+  if (cnt == 0 && wpk == 0) return;
+  Tally t;
+  ph_push<false>(D, make_range(), cascade_list(D, slot), cnt, kc, PM_ALL, nullptr, wlist(D, slot, wpk), t,
+                 Cascade{(slot + 1) % 3, stamp, relist});
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, int rmode,

@@ -10,9 +10,10 @@
 
 collect: for one set of training workloads, every batch is solved under every
 fixed schedule in {data, topo, pp} x {relabel every 1, 2, 3, 5, 8 rounds},
-data-driven ones also with the lazy start (no relabel before the first
-round), each from a push-pull static state so that every variant may take
-the batches.  Sets (WORKLOAD_SETS): base = R-MAT scale 15-20 with seeds 11-16
+data-driven ones with the engine's cascade push sub-rounds (as the learned
+scheduler runs them) and also with the lazy start (no relabel before the
+first round), each from a push-pull static state so that every variant may
+take the batches.  Sets (WORKLOAD_SETS): base = R-MAT scale 15-20 with seeds 11-16
 and 128/256 grids; large = R-MAT scale 20 with seeds 17 and 18 and scale 21
 with seed 19; grids = a 512 grid; xlarge = R-MAT scale 22 with seed 20;
 small = R-MAT scale 21 and 22 with seeds 21 and 22.  Batches are 1-10% of the
@@ -33,8 +34,9 @@ solve) and, from the features at the end of each round of the teacher's run,
 whether it relabelled before the next round.  A numpy MLP 8->32->32->4 (ReLU)
 is fitted with Adam; the weights are written in the row-major SchedWeights
 order of csrc/sched.hpp.  The shipped weights come from all five sets on the
-stepped driver (collect --mode 1), retrofitted, all variants, per-batch
-teachers.
+stepped driver (collect --mode 1), all variants, per-batch teachers; train
+--save-examples writes the examples it fitted (sched_examples.npz, committed)
+and train --examples refits from that file alone, reproducing the weights.
 """
 from __future__ import annotations
 
@@ -52,6 +54,10 @@ WEIGHTS = os.path.join(HERE, "csrc", "sched_weights.inc")
 FEAT, HID, OUT = 8, 32, 4
 VARIANTS = ["data", "topo", "pp"]
 INTERVALS = [1, 2, 3, 5, 8]
+# cascade sub-round cap of the data-driven teacher schedules and of the
+# static solves (the learned scheduler's default when its parameters give none)
+COLLECT_CASCADE = 1024
+EXAMPLES = os.path.join(HERE, "sched_examples.npz")
 
 
 WORKLOAD_SETS = {
@@ -121,14 +127,14 @@ def collect(out_dir: str, quick: bool = False, mode: int = 1, which: str = "base
         with dmf.Engine(g, s, t) as eng:
             os.environ.pop("DMF_TRACE", None)
             # push-pull static state: every variant may take the batches
-            base = dmf.default_params(g, variant="pp", mode=mode)
+            base = dmf.default_params(g, variant="pp", mode=mode, cascade=COLLECT_CASCADE)
             for variant in VARIANTS:
                 for interval in ([1, 3] if quick else INTERVALS):
                     for lazy in ((0, 1) if variant == "data" else (0,)):
                         eng.reset()
                         eng.static_solve(base)
                         p = dmf.default_params(g, variant=variant, relabel_interval=interval, lazy_start=lazy,
-                                               mode=mode)
+                                               mode=mode, cascade=COLLECT_CASCADE if variant == "data" else 0)
                         for src, dst, nw in batches:
                             eng.apply_batch(src, dst, nw, p)
                         print(f"[collect] {name} {variant} interval={interval} lazy={lazy}", flush=True)
@@ -200,6 +206,10 @@ def _examples(data, teacher="batch", variants=(0, 1, 2), variant_teacher="same")
             if not sm or sm["entry"] != 1 or sm["scheduler"] != 0:
                 continue
             if sm["variant"] not in variants:
+                continue
+            # data-driven teachers as the learned scheduler runs them: with
+            # cascade sub-rounds (traces that record none predate them)
+            if sm["variant"] == 0 and "cascade" in sm and sm["cascade"] <= 0:
                 continue
             key = (sm["variant"], sm["interval"], sm.get("lazy", 0))
             runs.setdefault(key, []).append(sv)
@@ -297,11 +307,20 @@ def write_weights(P, note: str) -> None:
         fh.write("\n".join(lines) + "\n")
 
 
-def train(trace_dirs, teacher="batch", variants=(0, 1, 2), variant_teacher="same") -> None:
-    data = {}
-    for d in trace_dirs:
-        data.update({f"{d}/{k}": v for k, v in _load(d).items()})
-    Xr, yr, Xv, yv = _examples(data, teacher, variants, variant_teacher)
+def train(trace_dirs, teacher="batch", variants=(0, 1, 2), variant_teacher="same", examples=None,
+          save_examples=None) -> None:
+    """From traces (trace_dirs), or from a saved example set (examples)."""
+    if examples:
+        with np.load(examples, allow_pickle=False) as z:
+            Xr, yr, Xv, yv = z["Xr"], z["yr"], z["Xv"], z["yv"]
+    else:
+        data = {}
+        for d in trace_dirs:
+            data.update({f"{d}/{k}": v for k, v in _load(d).items()})
+        Xr, yr, Xv, yv = _examples(data, teacher, variants, variant_teacher)
+    if save_examples:
+        np.savez_compressed(save_examples, Xr=Xr, yr=yr, Xv=Xv, yv=yv)
+        print(f"[train] wrote {save_examples}")
     print(f"[train] relabel examples {len(Xr)} (positive {yr.mean() if len(yr) else 0:.2f}), "
           f"variant examples {len(Xv)} (counts {np.bincount(yv, minlength=3).tolist()})")
     P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
@@ -326,7 +345,10 @@ def main():
     rf.add_argument("--traces", required=True)
     rf.add_argument("--out", required=True)
     t = sub.add_parser("train")
-    t.add_argument("--traces", required=True, nargs="+")
+    t.add_argument("--traces", nargs="+", default=[])
+    t.add_argument("--examples", default=None,
+                   help="fit from this saved example set instead of traces (e.g. the committed sched_examples.npz)")
+    t.add_argument("--save-examples", default=None, help="write the examples fitted to this .npz file")
     t.add_argument("--teacher", choices=["workload", "batch"], default="batch")
     t.add_argument("--variant-teacher", choices=["same", "workload"], default="same",
                    help="workload: each workload's best variant teaches all its batches")
@@ -340,7 +362,10 @@ def main():
             ap.error("--graph needs --source and --sink")
         collect(a.out, a.quick, a.mode, a.workloads, a.graph, a.source, a.sink)
     elif a.cmd == "train":
-        train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")), a.variant_teacher)
+        if not a.traces and not a.examples:
+            ap.error("train needs --traces or --examples")
+        train(a.traces, a.teacher, tuple(VARIANTS.index(x) for x in a.variants.split(",")), a.variant_teacher,
+              a.examples, a.save_examples)
 
 
 if __name__ == "__main__":

@@ -164,6 +164,14 @@ typedef struct dmf_params {
                                global relabel before the first round when the
                                previous solve ended on fresh labels.  The
                                learned scheduler makes this choice itself.    */
+  int32_t cascade;          /* data-driven rounds (stepped drivers): most push
+                               sub-rounds chained on the device after each
+                               round's push, each taking only the vertices
+                               the pass before it gave excess, and relabels
+                               that stop once every active vertex is labelled
+                               (convergence is still checked on a full one).
+                               0 = none, the reference's rounds; the learned
+                               scheduler uses 1024 when this is 0.           */
 } dmf_params;
 
 /* Fills the reference's fixed schedule: data variant, kc = max(1, m/n),

@@ -527,3 +527,54 @@ def test_early_exit_relabel_keeps_results(gpu, mode, interval, monkeypatch):
             applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1300 + b))
             f = eng.apply_batch(*applied[-1], p)
             check_engine(eng, g, caps_after(g, applied), s, t, f, f"early batch {b}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("variant", ["data", "pp"])
+def test_cascade_subrounds_keep_results(gpu, mode, variant, monkeypatch):
+    """Push sub-rounds chained on the device after each data-driven round
+    (forced on here for every schedule) end exact, together with the learned
+    scheduler and early-exit relabels."""
+    dmf = gpu
+    monkeypatch.setenv("DMF_CASCADE", "8")
+    g, s, t, (u, v, w) = rmat_case(12, 16, 14)
+    say(f"[cascade] variant={variant} mode={mode}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(dmf.default_params(g, variant=variant, mode=mode))
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        cur = w.copy()
+        applied = []
+        for b, pct in enumerate((1, 4, 2, 8, 3)):
+            applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1400 + b))
+            p = (dmf.default_params(g, variant=variant, mode=mode) if b % 2 == 0
+                 else dmf.default_params(g, mode=mode, scheduler=dmf.SCHED_LEARNED))
+            f = eng.apply_batch(*applied[-1], p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"cascade batch {b}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_cascade_parameter_on_rmat_and_grid(gpu, mode):
+    """The cascade parameter (sub-rounds chained on the device, launched in
+    checked chunks) under the fixed schedule and under the learned scheduler
+    (which uses it by default), on an R-MAT graph and on a grid where excess
+    travels far: exact after every batch."""
+    dmf = gpu
+    cases = [rmat_case(11, 16, 15)]
+    u, v, w = dmf.gen_grid(40, 40, 6, 1, 1000)
+    gg = dmf.build_csr(u, v, w)
+    cases.append((gg, 0, gg.n - 1, (u, v, w)))
+    for g, s, t, (u, v, w) in cases:
+        for sched in ("fixed", "learned"):
+            p = (dmf.default_params(g, mode=mode, cascade=64) if sched == "fixed"
+                 else dmf.default_params(g, mode=mode, scheduler=dmf.SCHED_LEARNED))
+            say(f"[cascade param] n={g.n} {sched} mode={mode}")
+            with dmf.Engine(g, s, t) as eng:
+                f = eng.static_solve(p)
+                check_engine(eng, g, g.cap, s, t, f, "static")
+                cur = w.copy()
+                applied = []
+                for b, pct in enumerate((2, 6, 3)):
+                    hi = 1000 if g is cases[1][0] else 100
+                    applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1500 + b, cap_hi=hi))
+                    f = eng.apply_batch(*applied[-1], p)
+                    check_engine(eng, g, caps_after(g, applied), s, t, f, f"{sched} batch {b}")

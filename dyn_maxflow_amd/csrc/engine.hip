@@ -169,12 +169,6 @@ int sync_ctrl(dmf_engine* E) {
   return DMF_OK;
 }
 
-int read_int(dmf_engine* E, const int32_t* dptr, int32_t* out) {
-  HIP_OK(hipMemcpyAsync(out, dptr, sizeof(int32_t), hipMemcpyDeviceToHost, E->stream));
-  HIP_OK(hipStreamSynchronize(E->stream));
-  return DMF_OK;
-}
-
 #define LAUNCH(kernel, blocks, ...)                                                   \
   do {                                                                                \
     hipLaunchKernelGGL(kernel, dim3(std::min<int>((blocks), E->grid_cap)), dim3(kBlock), 0, E->stream, \
@@ -517,7 +511,8 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       ++round;
       ++since;
       if (consult) {
-        if ((rc = run_sched(E, 1, since, "round", round, C.variant, fresh ? 1 : 0, active))) return rc;
+        // the trace records whether a relabel (full or early-exit) preceded the round
+        if ((rc = run_sched(E, 1, since, "round", round, C.variant, due ? 1 : 0, active))) return rc;
         want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
       }
     }
@@ -1517,39 +1512,32 @@ int dmf_engine_flow(dmf_engine* E, int64_t* flow) {
 int dmf_engine_min_cut(dmf_engine* E, uint8_t* side) {
   return guarded([&]() -> int {
     if (!E || !side) return DMF_EINVAL;
-    Dev& D = E->D;
-    // scratch between solves: the worklists, not order[] (a lazily started
-    // batch begins from the labelled set order[0, visited) of the last solve)
-    int32_t* mark = D.wl2;
-    int32_t* queue = D.wl;
-    int rc = DMF_OK;
-    do {
-      if ((rc = zero_dev(E, D.ctrl->qcnt, sizeof(D.ctrl->qcnt)))) break;
-      LAUNCH(k_cut_init, blocks_for(E->n), D, mark, queue);
-      int lo = 0, lvl = 0;
-      for (;;) {
-        int32_t cnt = 0;
-        if ((rc = read_int(E, &D.ctrl->qcnt[lvl % 3], &cnt))) break;
-        if (cnt == 0) break;
-        if ((rc = zero_dev(E, &D.ctrl->qcnt[(lvl + 1) % 3], sizeof(int32_t)))) break;
-        LAUNCH(k_cut_level, blocks_for(cnt), D, mark, queue, lo, cnt, (lvl + 1) % 3);
-        lo += cnt;
-        ++lvl;
-      }
-      if (rc) break;
-      std::vector<int32_t> h(E->n);
-      if (hipMemcpyAsync(h.data(), mark, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream) != hipSuccess ||
-          hipStreamSynchronize(E->stream) != hipSuccess) {
-        rc = DMF_EDEVICE;
-        break;
-      }
-      if (E->h_vmap.empty()) {
-        for (int32_t v = 0; v < E->n; ++v) side[v] = h[v] ? 1 : 0;
-      } else {
-        for (int32_t v = 0; v < E->n; ++v) side[v] = h[E->h_vmap[v]] ? 1 : 0;
-      }
-    } while (false);
-    return rc;
+    // The search is a BFS_CUT relabel on a copy of the device state whose
+    // labels, visit order and 16-bit filter are the worklists (scratch
+    // between solves), so the labels and labelled set order[0, visited) that
+    // a lazily started batch reuses stay as they are.
+    const Dev saved = E->D;
+    const dmf_counters last = E->last;
+    const int last_levels = E->last_levels;
+    int32_t* mark = E->D.wl2;
+    E->D.height = mark;
+    E->D.order = E->D.wl;
+    E->D.lev16 = reinterpret_cast<uint16_t*>(E->D.part);  // 4n bytes hold n 16-bit labels
+    int visited = 0;
+    int rc = relabel(E, BFS_CUT, 0, &visited);
+    E->D = saved;
+    E->last = last;
+    E->last_levels = last_levels;
+    if (rc) return rc;
+    std::vector<int32_t> h(E->n);
+    HIP_OK(hipMemcpyAsync(h.data(), mark, sizeof(int32_t) * E->n, hipMemcpyDeviceToHost, E->stream));
+    HIP_OK(hipStreamSynchronize(E->stream));
+    if (E->h_vmap.empty()) {
+      for (int32_t v = 0; v < E->n; ++v) side[v] = h[v] < E->n ? 1 : 0;
+    } else {
+      for (int32_t v = 0; v < E->n; ++v) side[v] = h[E->h_vmap[v]] < E->n ? 1 : 0;
+    }
+    return DMF_OK;
   });
 }
 

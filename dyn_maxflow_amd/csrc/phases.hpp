@@ -595,8 +595,10 @@ __device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
 // which a deficit pulls flow back; BFS_CS labels the distance to the sources
 // in the residual graph, along which stuck excess is returned.  Neither passes
 // through a terminal.
-enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2, BFS_CT = 3, BFS_CS = 4 };
-constexpr int kBfsModes = 5;
+// BFS_CUT is the S_min search: reachability over res > 0 from the sources and
+// every non-sink vertex with positive excess.
+enum BfsMode { BFS_SINK = 0, BFS_PP = 1, BFS_FINAL = 2, BFS_CT = 3, BFS_CS = 4, BFS_CUT = 5 };
+constexpr int kBfsModes = 6;
 
 __device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1] - D.row[v]; }
 
@@ -626,6 +628,8 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t, bo
         root = is_snk(D, v) || (!is_term(D, v) && ex > 0);
       } else if (mode == BFS_CS) {
         root = is_src(D, v);
+      } else if (mode == BFS_CUT) {
+        root = is_src(D, v) || (!is_snk(D, v) && ex > 0);
       } else {
         const bool inB = D.side[v] != 0;
         root = is_term(D, v) || (inB && ex < 0) || (!inB && ex > 0);
@@ -719,11 +723,11 @@ __device__ __forceinline__ int bfs_resid(const Dev& D, int mode, bool vB, int e)
     const int32_t back = D.rres[e], fwd = D.res[e];
     return vB ? back : fwd;
   }
-  if (mode == BFS_CT) return D.res[e];  // mode is uniform over the launch
+  if (mode == BFS_CT || mode == BFS_CUT) return D.res[e];  // mode is uniform over the launch
   return D.rres[e];
 }
 __device__ __forceinline__ bool bfs_head_ok(const Dev& D, int mode, int w) {
-  if (mode == BFS_SINK) return true;
+  if (mode == BFS_SINK || mode == BFS_CUT) return true;
   if (mode == BFS_PP || mode == BFS_CT || mode == BFS_CS) return !is_term(D, w);
   return !D.side[w] && D.pfinal[w];
 }
@@ -1748,74 +1752,6 @@ __device__ void ph_upd_release(const Dev& D, const Range& r) {
   }
 }
 
-// Reachability from s and from positive-excess vertices over res > 0 arcs
-// (S_min), one BFS level.  mark and queue are caller scratch arrays of n ints
-// (the queue holds each reached vertex once, frontiers as contiguous slices);
-// neither is order[], whose labelled set a lazily started batch reuses.
-__device__ void ph_cut_init(const Dev& D, const Range& r, int32_t* mark, int32_t* queue) {
-  for (int base = r.gid - lane_id(); base < D.n; base += r.gsz) {
-    const int v = base + lane_id();
-    bool root = false;
-    if (v < D.n) {
-      root = is_src(D, v) || (!is_snk(D, v) && D.excess[v] > 0);
-      mark[v] = root ? 1 : 0;
-    }
-    int rank;
-    const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
-    if (root) queue[off + rank] = v;
-  }
-}
-
-__device__ void ph_cut_level(const Dev& D, const Range& r, int32_t* mark, int32_t* queue, int lo, int cnt,
-                             int32_t* next_cnt) {
-  int32_t* out = queue + lo + cnt;
-  for (int base = r.wid * kWave; base < cnt; base += r.nw * kWave) {
-    const int i = base + lane_id();
-    int b = 0, deg = 0;
-    if (i < cnt) {
-      const int v = queue[lo + i];
-      b = D.row[v];
-      deg = D.row[v + 1] - b;
-    }
-    unsigned long long big = __ballot(deg > D.lane_max);
-    while (big) {
-      const int leader = __ffsll(static_cast<long long>(big)) - 1;
-      big &= big - 1;
-      const int bb = __shfl(b, leader);
-      const int bd = __shfl(deg, leader);
-      for (int j0 = 0; j0 < bd; j0 += kWave) {
-        const int j = j0 + lane_id();
-        bool hit = false;
-        int w = -1;
-        if (j < bd) {
-          const int e = bb + j;
-          w = D.col[e];
-          if (D.res[e] > 0 && mark[w] == 0) hit = atomicCAS(&mark[w], 0, 1) == 0;
-        }
-        int rank;
-        const int off = wave_reserve(next_cnt, hit, &rank);
-        if (hit) out[off + rank] = w;
-      }
-    }
-    const int my_deg = deg <= D.lane_max ? deg : 0;
-    int maxd = my_deg;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
-    for (int j = 0; j < maxd; ++j) {
-      bool hit = false;
-      int w = -1;
-      if (j < my_deg) {
-        const int e = b + j;
-        w = D.col[e];
-        if (D.res[e] > 0 && mark[w] == 0) hit = atomicCAS(&mark[w], 0, 1) == 0;
-      }
-      int rank;
-      const int off = wave_reserve(next_cnt, hit, &rank);
-      if (hit) out[off + rank] = w;
-    }
-  }
-}
-
 // ============================================================ stepped kernels
 // Control-block clears done by kernels rather than hipMemsetAsync: each is one
 // small launch in place of several runtime fill dispatches.
@@ -2070,13 +2006,6 @@ __global__ void __launch_bounds__(kBlock) k_upd_clamp(Dev D) {
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_upd_release(Dev D) { ph_upd_release(D, make_range()); }
-__global__ void __launch_bounds__(kBlock) k_cut_init(Dev D, int32_t* mark, int32_t* queue) {
-  ph_cut_init(D, make_range(), mark, queue);
-}
-__global__ void __launch_bounds__(kBlock) k_cut_level(Dev D, int32_t* mark, int32_t* queue, int lo, int cnt,
-                                                      int slot) {
-  ph_cut_level(D, make_range(), mark, queue, lo, cnt, &D.ctrl->qcnt[slot]);
-}
 __global__ void __launch_bounds__(kBlock) k_build_imbalance_wl(Dev D, int cnt, int deficits) {
   Tally t;
   ph_build_imbalance_wl(D, make_range(), cnt, deficits != 0, t);

@@ -294,7 +294,7 @@ def fit(Xr, yr, Xv, yv, steps=4000, lr=3e-3, seed=0):
     return P, acc_r, acc_v
 
 
-def write_weights(P, note: str) -> None:
+def write_weights(P, note: str, path: str | None = None) -> None:
     flat = np.concatenate([P["w1"].ravel(), P["b1"], P["w2"].ravel(), P["b2"], P["w3"].ravel(), P["b3"]])
     lines = ["// Scheduler weights for sched.hpp, row-major in SchedWeights order.",
              f"// Trained by dyn_maxflow_amd/sched_train.py: {note}",
@@ -303,12 +303,12 @@ def write_weights(P, note: str) -> None:
     for i in range(0, len(flat), 6):
         lines.append("  " + ", ".join(f"{float(x)!r}f" for x in flat[i:i + 6]) + ",")
     lines.append("};")
-    with open(WEIGHTS, "w") as fh:
+    with open(path or WEIGHTS, "w") as fh:
         fh.write("\n".join(lines) + "\n")
 
 
 def train(trace_dirs, teacher="batch", variants=(0, 1, 2), variant_teacher="same", examples=None,
-          save_examples=None) -> None:
+          save_examples=None, out=None) -> None:
     """From traces (trace_dirs), or from a saved example set (examples)."""
     if examples:
         with np.load(examples, allow_pickle=False) as z:
@@ -326,8 +326,8 @@ def train(trace_dirs, teacher="batch", variants=(0, 1, 2), variant_teacher="same
     P, acc_r, acc_v = fit(Xr, yr, Xv, yv)
     print(f"[train] accuracy: relabel {acc_r:.3f} variant {acc_v:.3f}")
     write_weights(P, f"{len(Xr)} relabel / {len(Xv)} variant examples, train acc "
-                     f"{acc_r:.3f} / {acc_v:.3f}")
-    print(f"[train] wrote {WEIGHTS}")
+                     f"{acc_r:.3f} / {acc_v:.3f}", out)
+    print(f"[train] wrote {out or WEIGHTS}")
 
 
 def main():

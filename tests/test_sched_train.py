@@ -114,3 +114,13 @@ def test_retrofit_sets_graph_size_at_batch_start(tmp_path):
             assert r["feat"][0] == want and r["feat"][1:] == o["feat"][1:]
         else:
             assert r == o  # round and summary rows are unchanged
+
+
+def test_committed_examples_rebuild_the_shipped_weights(tmp_path):
+    """The shipped weights (csrc/sched_weights.inc) are the fit of the committed
+    example set (sched_examples.npz, from the engine's own traces): refitting
+    from that file alone writes the same file byte for byte."""
+    from dyn_maxflow_amd import sched_train as st
+    out = tmp_path / "w.inc"
+    st.train([], examples=st.EXAMPLES, out=str(out))
+    assert out.read_bytes() == open(st.WEIGHTS, "rb").read()

@@ -395,8 +395,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the profiled pass (the command its rocprofv3 summaries are taken from)")
     ap.add_argument("--only", default=None,
-                    help="run only the plain pass of this variant (data, pp or topo) and print its object "
-                         "(the command the per-variant rocprofv3 summaries are taken from)")
+                    help="run only one pass (data, pp, topo, data+cascade or learned) and print its object "
+                         "(the command per-pass rocprofv3 summaries are taken from)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (PMC scaling reference) and exit")
     args = ap.parse_args()
@@ -420,9 +420,14 @@ def main():
         return dmf.default_params(g, variant=variant, mode=args.mode, relabel_interval=args.interval)
 
     if args.only:
-        recs, _ = run_gpu(g, s, t, batches, plain_params(args.only), args.steps, args.warmup, f"plain {args.only}",
-                          static_params=static)
-        print(json.dumps({"variant": args.only, "plain": pass_object(summarize(recs))}), flush=True)
+        if args.only == "learned":
+            p1 = dmf.default_params(g, mode=args.mode if args.mode else 1, scheduler=dmf.SCHED_LEARNED)
+        else:
+            p1 = plain_params(args.only.split("+")[0])
+            if args.only.endswith("+cascade"):
+                p1.cascade = args.cascade
+        recs, _ = run_gpu(g, s, t, batches, p1, args.steps, args.warmup, args.only, static_params=static)
+        print(json.dumps({"pass": args.only, "summary": pass_object(summarize(recs))}), flush=True)
         return
     # Profiled pass: the plain schedule on the bench's driver, with every level
     # launch (graph driver: every replayed segment of them) timed by HIP events.

@@ -101,6 +101,16 @@ struct dmf_engine {
   int cascade_kc = 1;
   int cascade_relist = 0;
   int32_t cstamp = 0;
+  int32_t sched_tag = 0;  // tag of the last round whose decision a cascade launch takes
+  // Two buffers for the BFS visit order: a restricted relabel tries as
+  // bottom-up candidates only the labelled set of the solve's last complete
+  // relabel from t (ord[full_buf], full_n vertices) and writes the other
+  // buffer.  full_buf < 0: no such set yet in this solve.
+  int32_t* ord[2] = {nullptr, nullptr};
+  int full_buf = -1;
+  int full_n = 0;
+  int restrict_mode = 4;  // DMF_BFS_RESTRICT: 0 off, else Dev::bu_rest
+  bool restricted = false;  // the last relabel ran a restricted bottom-up level
   int visited_last = 0;
   FILE* trace = nullptr;          // per-round scheduler trace (DMF_TRACE)
   int32_t ninst = 1;              // independent instances stored as one disjoint union
@@ -218,7 +228,7 @@ int relabel_stepped(dmf_engine* E, int mode, int part_cnt, int* visited, bool ea
     const int32_t cnt = E->h_ctrl->qcnt[lvl % 3];
     const unsigned long long wpk = E->h_ctrl->wq[lvl % 3];
     if (cnt == 0) break;
-    const int bu = use_bottom_up(D, mode, cnt, chunks_of(wpk)) ? 1 : 0;
+    const int bu = use_bottom_up(D, mode, lo, cnt, chunks_of(wpk)) ? 1 : 0;
     const int grid = std::min(bu ? blocks_for(E->n) : bfs_blocks(cnt, chunks_of(wpk)), E->grid_cap);
     if (E->profile) {
       // events stamped by the launch itself: the span is the kernel's execution
@@ -325,8 +335,7 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
 
 // early (BFS_SINK only): stop once every vertex active now is labelled; the
 // control block's bfs_cut then tells that the labels are not complete.
-int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = false) {
-  early = early && mode == BFS_SINK;
+int relabel_run(dmf_engine* E, int mode, int part_cnt, int* visited, bool early) {
   if (E->use_graph && !E->graph_failed) {
     const int rc = relabel_graph(E, mode, part_cnt, visited, early);
     if (rc != kNoGraph) return rc;
@@ -336,6 +345,37 @@ int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = fa
     std::fprintf(stderr, "dmf: BFS graph capture unavailable, using per-level launches\n");
   }
   return relabel_stepped(E, mode, part_cnt, visited, early);
+}
+
+// Global relabel.  restrict_prev (from t only): bottom-up levels try only the
+// labelled set of the solve's last complete relabel from t.  A vertex that
+// became able to reach t since then is missed by such a level and keeps
+// height n, so a relabel that ran one is not taken as fresh (E->restricted;
+// its caller confirms convergence with an unrestricted relabel).  Relabels
+// from t write the order buffer that does not hold that set, and a complete
+// one (not cut short by an early exit) becomes the set for the next.
+int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = false, bool restrict_prev = false) {
+  early = early && mode == BFS_SINK;
+  Dev& D = E->D;
+  const bool two = mode == BFS_SINK && E->ord[1] != nullptr;
+  const bool rs = two && restrict_prev && E->full_buf >= 0 && E->full_n > 0 && E->restrict_mode > 0;
+  if (two && E->full_buf >= 0) D.order = E->ord[E->full_buf ^ 1];
+  if (rs) {
+    D.prev = E->ord[E->full_buf];
+    D.nprev = E->full_n;
+    D.bu_rest = E->restrict_mode;
+  }
+  const int rc = relabel_run(E, mode, part_cnt, visited, early);
+  D.prev = nullptr;
+  D.nprev = 0;
+  // labels of a restricted relabel whose levels all ran top-down are exact
+  E->restricted = rs && E->h_ctrl->bfs_rbu != 0;
+  if (rc) return rc;
+  if (two && !(early && E->h_ctrl->bfs_cut)) {
+    E->full_buf = D.order == E->ord[1] ? 1 : 0;
+    E->full_n = *visited;
+  }
+  return DMF_OK;
 }
 
 int reset_lists(dmf_engine* E) {
@@ -350,28 +390,44 @@ double since_solve_ms(const dmf_engine* E) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - E->t_solve).count();
 }
 
-// Evaluates the scheduler MLP on the device and reads its decision back.
-int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int round, int variant,
-              int relabelled, double active) {
+// One line of the scheduler trace (DMF_TRACE) for the decision in h_sched.
+void sched_trace(dmf_engine* E, const char* what, int round, int variant, int relabelled) {
+  if (!E->trace) return;
+  const SchedState& S = *E->h_sched;
+  std::fprintf(E->trace,
+               "{\"solve\":%llu,\"what\":\"%s\",\"round\":%d,\"variant\":%d,\"relabelled\":%d,"
+               "\"k\":%d,\"t_ms\":%.4f,\"feat\":[%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g],"
+               "\"out\":[%.6g,%.6g,%.6g,%.6g]}\n",
+               (unsigned long long)E->solve_id, what, round, variant, relabelled, E->D.k, since_solve_ms(E),
+               S.feat[0], S.feat[1], S.feat[2], S.feat[3], S.feat[4], S.feat[5], S.feat[6], S.feat[7],
+               S.out[0], S.out[1], S.out[2], S.out[3]);
+}
+
+// Fraction of the batch's updates that raise a capacity (scheduler feature).
+float inc_fraction(const dmf_engine* E) {
+  const int k = E->D.k;
+  return k > 0 ? static_cast<float>(E->h_ctrl->inc) / static_cast<float>(k) : 0.5f;
+}
+
+// Queues the scheduler MLP on the device and the copy of its decision to the
+// host (inc < 0: the kernel reads the batch's increase count itself).
+int launch_sched(dmf_engine* E, int phase, int rounds_since, double active, float inc) {
   Dev& D = E->D;
-  const int k = D.k;
-  const float inc = k > 0 ? static_cast<float>(E->h_ctrl->inc) / static_cast<float>(k) : 0.5f;
-  hipLaunchKernelGGL(k_sched, dim3(1), dim3(kWave), 0, E->stream, D, E->d_sched_w, E->d_sched, phase, k, inc,
+  hipLaunchKernelGGL(k_sched, dim3(1), dim3(kWave), 0, E->stream, D, E->d_sched_w, E->d_sched, phase, D.k, inc,
                      rounds_since, E->last_levels, active);
   ++E->launches;
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(E->h_sched, E->d_sched, sizeof(SchedState), hipMemcpyDeviceToHost, E->stream));
+  return DMF_OK;
+}
+
+// Evaluates the scheduler MLP on the device and reads its decision back.
+int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int round, int variant,
+              int relabelled, double active) {
+  int rc;
+  if ((rc = launch_sched(E, phase, rounds_since, active, inc_fraction(E)))) return rc;
   HIP_OK(hipStreamSynchronize(E->stream));
-  if (E->trace) {
-    const SchedState& S = *E->h_sched;
-    std::fprintf(E->trace,
-                 "{\"solve\":%llu,\"what\":\"%s\",\"round\":%d,\"variant\":%d,\"relabelled\":%d,"
-                 "\"k\":%d,\"t_ms\":%.4f,\"feat\":[%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g,%.6g],"
-                 "\"out\":[%.6g,%.6g,%.6g,%.6g]}\n",
-                 (unsigned long long)E->solve_id, what, round, variant, relabelled, k, since_solve_ms(E),
-                 S.feat[0], S.feat[1], S.feat[2], S.feat[3], S.feat[4], S.feat[5], S.feat[6], S.feat[7],
-                 S.out[0], S.out[1], S.out[2], S.out[3]);
-  }
+  sched_trace(E, what, round, variant, relabelled);
   return DMF_OK;
 }
 
@@ -381,6 +437,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   int rc;
   const bool labels_in = E->labels_fresh;
   E->labels_fresh = false;  // set again only by a converged data-driven loop
+  E->full_buf = -1;         // the update may have let more vertices reach t
   Cfg C = C0;
   const bool learned = C.scheduler == DMF_SCHED_LEARNED;
   const bool consult = learned || E->trace != nullptr;
@@ -392,15 +449,17 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     if (C.variant == DMF_VARIANT_PP) LAUNCH(k_saturate_sink_pp, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
   } else if (C.entry == 1) {
     LAUNCH(k_upd_lookup, blocks_for(D.k), D);
+    // batch start: the scheduler picks the variant (push-pull only from a
+    // push-pull static state); its decision comes back with the lookup's
+    // counters in one host check
+    if (consult && (rc = launch_sched(E, 0, 0, 0.0, -1.f))) return rc;
     if ((rc = sync_ctrl(E))) return rc;
     if (E->h_ctrl->missing || E->h_ctrl->invalid) {  // caller reports; owners cleared
       LAUNCH(k_upd_release, blocks_for(D.k), D);
       return DMF_OK;
     }
     if (consult) {
-      // batch start: the scheduler picks the variant (push-pull only from a
-      // push-pull static state)
-      if ((rc = run_sched(E, 0, 0, "batch", 0, C.variant, 0, 0.0))) return rc;
+      sched_trace(E, "batch", 0, C.variant, 0);
       if (learned && E->h_sched->valid && (E->h_sched->variant != DMF_VARIANT_PP || E->pp_ready))
         C.variant = E->h_sched->variant;
     }
@@ -449,15 +508,19 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       bool fresh = false;
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
+        // the relabels that may stop early may also restrict their bottom-up
+        // levels to the last complete labelled set
         const bool early = try_early && !need_full;
-        if ((rc = relabel(E, BFS_SINK, 0, &visited, early))) return rc;
+        if ((rc = relabel(E, BFS_SINK, 0, &visited, early, early))) return rc;
         since = 0;
-        fresh = !(early && E->h_ctrl->bfs_cut);
-        if (!fresh) LAUNCH(k_cap_unlabelled, blocks_for(n), D, std::max(0, E->last_levels - 1));
+        const bool cut = early && E->h_ctrl->bfs_cut;
+        fresh = !cut && !E->restricted;
+        if (cut) LAUNCH(k_cap_unlabelled, blocks_for(n), D, std::max(0, E->last_levels - 1));
         need_full = false;
       }
       int32_t cnt = 0;
       unsigned long long wpk = 0;
+      bool sched_done = false;  // the cascade chain took the round-end decision
       if (!topo) {
         // the list counters were cleared by the last push launch (or k_begin)
         LAUNCH(k_build_wl, blocks_for(visited), D, visited);
@@ -484,6 +547,11 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         if (!E->h_ctrl->flag[fs]) break;
         active = static_cast<double>(n);
       } else {
+        if (sub > 0 && E->cstamp > INT_MAX - (sub + 2)) {
+          // list stamps only grow: start them again from a cleared mark array
+          if ((rc = zero_dev(E, D.cmark, sizeof(int32_t) * static_cast<size_t>(n)))) return rc;
+          E->cstamp = 0;
+        }
         LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
                sub ? ++E->cstamp : 0, E->cascade_relist);
         LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
@@ -491,20 +559,29 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         // cascade sub-rounds: each pushes from the list the pass before it
         // wrote, launched kCascadeChunk at a time with one host check between
         // chunks (an empty list ends the chain)
+        const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
         if (sub > 0) {
           const int cgrid = std::max(1, std::min(E->grid_cap, 2 * std::max(1, E->num_cu)));
-          const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
+          // the scheduler's round-end decision rides on the first launch
+          // that finds the chain's list empty
+          const SchedWeights* sw = consult ? E->d_sched_w : nullptr;
+          const SchedRound sr{D.k, inc_fraction(E), since + 1, E->last_levels, active, ++E->sched_tag};
           int slot = 0;
           for (int done_sub = 0; done_sub < sub;) {
             const int chunk = std::min(kCascadeChunk, sub - done_sub);
             for (int j = 0; j < chunk; ++j) {
-              LAUNCH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist);
+              LAUNCH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist, sw, E->d_sched, sr);
               slot = (slot + 1) % 3;
             }
             done_sub += chunk;
             if (done_sub >= sub) break;
+            if (consult)
+              HIP_OK(hipMemcpyAsync(E->h_sched, E->d_sched, sizeof(SchedState), hipMemcpyDeviceToHost, E->stream));
             if ((rc = sync_ctrl(E))) return rc;
-            if (E->h_ctrl->qcnt[slot] == 0 && E->h_ctrl->wq[slot] == 0) break;
+            if (E->h_ctrl->qcnt[slot] == 0 && E->h_ctrl->wq[slot] == 0) {
+              sched_done = consult && E->h_sched->tag == sr.tag;
+              break;
+            }
           }
         }
       }
@@ -512,7 +589,8 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       ++since;
       if (consult) {
         // the trace records whether a relabel (full or early-exit) preceded the round
-        if ((rc = run_sched(E, 1, since, "round", round, C.variant, due ? 1 : 0, active))) return rc;
+        if (sched_done) sched_trace(E, "round", round, C.variant, due ? 1 : 0);
+        else if ((rc = run_sched(E, 1, since, "round", round, C.variant, due ? 1 : 0, active))) return rc;
         want_relabel = learned ? (!E->h_sched->valid || E->h_sched->relabel != 0) : true;
       }
     }
@@ -1053,6 +1131,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   uint16_t* lev16 = nullptr;
   uint8_t* tgt = nullptr;
   int32_t* cmark = nullptr;
+  int32_t* order2 = nullptr;
   int rc = DMF_OK;
   const bool alloc_failed = (rc = dalloc(E, &row, n + 1)) || (rc = dalloc(E, &col, m)) || (rc = dalloc(E, &rev, m)) ||
       (rc = dalloc(E, &cap, m)) || (rc = dalloc(E, &E->d_cap0, m)) || (rc = dalloc(E, &res, m)) ||
@@ -1063,7 +1142,8 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
-      (rc = dalloc(E, &lev16, n)) || (rc = dalloc(E, &tgt, n)) || (rc = dalloc(E, &cmark, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
+      (rc = dalloc(E, &lev16, n)) || (rc = dalloc(E, &tgt, n)) || (rc = dalloc(E, &cmark, n)) ||
+      (rc = dalloc(E, &order2, n)) || (inst && (rc = dalloc(E, &d_inst, n))) ||
       (!E->h_vmap.empty() && (rc = dalloc(E, &E->d_vmap, n)));
   D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
   D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
@@ -1073,6 +1153,8 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   D.lev16 = lev16;
   D.tgt = tgt;
   D.cmark = cmark;
+  E->ord[0] = order;
+  E->ord[1] = order2;
   D.vmap = E->d_vmap;
   if (alloc_failed) {
     dmf_engine_destroy(E);
@@ -1088,6 +1170,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
     if (const char* cs = std::getenv("DMF_CASCADE")) E->cascade_env = std::max(0, std::atoi(cs));
     if (const char* ck = std::getenv("DMF_CASCADE_KC")) E->cascade_kc = std::max(0, std::atoi(ck));
     if (const char* cr = std::getenv("DMF_CASCADE_RELIST")) E->cascade_relist = std::atoi(cr);
+    if (const char* br = std::getenv("DMF_BFS_RESTRICT")) E->restrict_mode = std::max(0, std::atoi(br));
   }
   if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
     dmf_engine_destroy(E);
@@ -1375,7 +1458,8 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(D.rres);
     hipFree(D.excess);
     hipFree(D.height);
-    hipFree(D.order);
+    hipFree(E->ord[0]);
+    hipFree(E->ord[1]);
     hipFree(D.wl);
     hipFree(D.wl2);
     hipFree(D.part);
@@ -1622,6 +1706,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += 2 * N;                                   // 16-bit BFS levels
   b += N;                                       // early-exit relabel targets
   b += 4 * N;                                   // cascade list stamps
+  b += 4 * N;                                   // second BFS order buffer
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)

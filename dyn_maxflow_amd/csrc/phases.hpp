@@ -80,6 +80,7 @@ struct Ctrl {
   int32_t inc;        // updates that raise an arc's capacity (scheduler feature)
   int32_t early;      // the current relabel may stop once its targets are labelled
   int32_t bfs_cut;    // the current relabel stopped before its frontier ran out
+  int32_t bfs_rbu;    // the current relabel ran a restricted bottom-up level
   int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
                       // read from slot r and written to slot r^1 by each level
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
@@ -118,6 +119,14 @@ struct Dev {
   unsigned long long* ctrs;  // C_N x kSlots work counters
   uint16_t* lev16;           // BFS filter: 0xFFFF unlabelled in the current relabel, else min(level, 0xFFFE)
   int32_t bottom_up;         // bottom-up BFS level once frontier arcs > m / bottom_up (0 = never)
+  // Restricted bottom-up levels (relabels from t inside a solve): prev[0,
+  // nprev) is the labelled set of the solve's last complete relabel, the only
+  // vertices a bottom-up level then tries; one is taken once the frontier's
+  // arcs exceed bu_rest times the candidates not yet labelled.  nprev = 0: no
+  // restriction (the rule above applies).
+  const int32_t* prev;
+  int32_t nprev;
+  int32_t bu_rest;
   unsigned long long* hkey;
   int32_t* hval;
   unsigned long long hmask;
@@ -863,40 +872,49 @@ __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int
 }
 
 // Bottom-up BFS level over the reverse residual graph (same levels as the
-// top-down rule of staticMaxFlow_kernel_11): every unlabelled vertex w looks
+// top-down rule of staticMaxFlow_kernel_11): every unlabelled candidate w looks
 // for one arc w->x with residual > 0 into the current frontier (height ==
 // lvl) and stops at the first; no compare-and-swap is needed because w is
-// written only by its own lane.  Used for the global relabel from t when the
-// frontier is large.
+// written only by its own lane.  Candidates are every vertex (cand null) or
+// cand[0, ncand).  Used for the global relabel from t when the frontier is
+// large.  early: count the targets this level labels (early-exit relabel).
 __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* next_cnt, int qnext,
-                                int32_t* out, Tally& t) {
+                                int32_t* out, Tally& t, const int32_t* cand = nullptr, int ncand = 0,
+                                bool early = false) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
   Stage st{sbuf[threadIdx.x / kWave], 0};
   const int n = D.n;
   const int32_t* __restrict__ col = D.col;
   const int32_t* __restrict__ res = D.res;
   const int32_t* height = D.height;
-  for (int base = r.wid * kWave; base < n; base += r.nw * kWave) {
-    const int w = base + lane_id();
-    bool cand = false;
+  const int total = cand ? ncand : n;
+  long long found = 0;
+  if (cand && is_block0_thread0()) D.ctrl->bfs_rbu = 1;
+  for (int base = r.wid * kWave; base < total; base += r.nw * kWave) {
+    const int i = base + lane_id();
+    int w = -1;
+    bool cand_ok = false;
     int b = 0, deg = 0;
-    if (w < n && height[w] == n) {
-      b = D.row[w];
-      deg = D.row[w + 1] - b;
-      cand = deg > 0;
-      t.vert += 1;
+    if (i < total) {
+      w = cand ? cand[i] : i;
+      if (D.lev16[w] == kLevFree && height[w] == n) {
+        b = D.row[w];
+        deg = D.row[w + 1] - b;
+        cand_ok = deg > 0;
+        t.vert += 1;
+      }
     }
     bool hit = false;
     // candidates above lane_max: the whole wave scans, two 64-arc tiles per
     // step, and stops at the first tile holding a parent
-    unsigned long long big = __ballot(cand && deg > D.lane_max);
+    unsigned long long big = __ballot(cand_ok && deg > D.lane_max);
     while (big) {
       const int leader = __ffsll(static_cast<long long>(big)) - 1;
       big &= big - 1;
       const int bb = __shfl(b, leader);
       const int bd = __shfl(deg, leader);
-      bool found = false;
-      for (int j0 = 0; j0 < bd && !found; j0 += 2 * kWave) {
+      bool fnd = false;
+      for (int j0 = 0; j0 < bd && !fnd; j0 += 2 * kWave) {
         const int jA = j0 + lane_id(), jB = jA + kWave;
         const bool inA = jA < bd, inB = jB < bd;
         const int xA = inA ? col[bb + jA] : 0, rA = inA ? res[bb + jA] : 0;
@@ -904,13 +922,13 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
         const int hA = rA > 0 ? height[xA] : -1;
         const int hB = rB > 0 ? height[xB] : -1;
         t.arc += (inA ? 1 : 0) + (inB ? 1 : 0);
-        found = __ballot(hA == lvl || hB == lvl) != 0;
+        fnd = __ballot(hA == lvl || hB == lvl) != 0;
       }
-      if (lane_id() == leader) hit = found;
+      if (lane_id() == leader) hit = fnd;
     }
     // narrow candidates: each lane scans its own arcs, four independent
     // (col, res) pairs and height gathers in flight per step
-    if (cand && deg <= D.lane_max) {
+    if (cand_ok && deg <= D.lane_max) {
       const int end = b + deg;
       for (int e = b; e < end && !hit; e += 4) {
         const bool i1 = e + 1 < end, i2 = e + 2 < end, i3 = e + 3 < end;
@@ -931,8 +949,13 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
       D.height[w] = lvl + 1;
       D.lev16[w] = static_cast<uint16_t>(lvl + 1 < kLevFree ? lvl + 1 : kLevFree - 1);
       if (deg > D.wide_t) wq_append(D, qnext, w, deg);
+      if (early) found += D.tgt[w];
     }
     st.push(hit, w, next_cnt, out);
+  }
+  if (early) {
+    found = wave_sum_i64(found);
+    if (lane_id() == 0 && found) atomicAdd(&D.ctrs[C_TGTF * kSlots + (r.wid % kSlots)], found);
   }
   st.flush(next_cnt, out);
 }
@@ -943,10 +966,26 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
 // arcs are estimated as its narrow vertices times the mean degree plus its
 // wide chunks times kChunk.  R-MAT frontiers are small in vertices but hold
 // the hubs' arcs, so the estimate counts arcs, not vertices.
-__host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, int cnt, long long chunks) {
-  if (mode != BFS_SINK || D.bottom_up <= 0) return false;
+// Restricted (D.nprev > 0, the candidates are the last complete relabel's
+// labelled set): bottom-up once the frontier holds at least 1/bu_rest of the
+// candidates not yet labelled (labelled so far: lo + cnt, the roots and the
+// levels up to the current one) and more arcs than there are such
+// candidates.  Past that point most candidates left lie on the next level and
+// find a parent in their first few arcs; before it, most of them would scan
+// every arc and find none, which costs more than expanding the frontier even
+// when hubs make it heavy.  On a residual graph near a maximum flow the
+// unrestricted rule loses: the source side, unreachable, is scanned in full
+// by every bottom-up level; the restricted candidates leave it out.
+__host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, int lo, int cnt, long long chunks) {
+  if (mode != BFS_SINK) return false;
   const long long mean = D.n > 0 ? D.m / D.n : 0;
   const long long arcs = static_cast<long long>(cnt) * mean + chunks * kChunk;
+  if (D.nprev > 0) {
+    if (D.bu_rest <= 0) return false;
+    const long long left = static_cast<long long>(D.nprev) - lo - cnt;
+    return left > 0 && static_cast<long long>(cnt) * D.bu_rest >= left && arcs > left;
+  }
+  if (D.bottom_up <= 0) return false;
   return arcs * D.bottom_up > D.m;
 }
 
@@ -1127,6 +1166,45 @@ __device__ __forceinline__ void cascade_note(const Dev& D, const Cascade& cz, bo
   }
   if (wide) wq_append(D, cz.slot, x, deg);
 }
+// The same for up to N candidates per lane, with every stamp claim and
+// degree load of the N issued before any result is waited for, and one list
+// reservation for all of them: a lane that pushed along several arcs in one
+// step waits for one round trip per stage, not one per arc.
+template <int N>
+__device__ __forceinline__ void cascade_note_n(const Dev& D, const Cascade& cz, const bool (&cand)[N],
+                                               const int (&x)[N]) {
+  int m[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) m[j] = cand[j] ? atomicMax(&D.cmark[x[j]], cz.stamp) : cz.stamp;
+  int deg[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) deg[j] = m[j] < cz.stamp ? degree(D, x[j]) : -1;
+  unsigned long long bn[N];
+  unsigned long long any = 0;
+  int total = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    bn[j] = __ballot(deg[j] >= 0 && deg[j] <= D.lane_max);
+    any |= bn[j];
+    total += __popcll(bn[j]);
+  }
+  if (any) {
+    const int leader = __ffsll(static_cast<long long>(any)) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(&D.ctrl->qcnt[cz.slot], total);
+    base = __shfl(base, leader);
+    int32_t* list = cascade_list(D, cz.slot);
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if ((bn[j] >> lane_id()) & 1ull) list[base + __popcll(bn[j] & below)] = x[j];
+      base += __popcll(bn[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (deg[j] > D.lane_max) wq_append(D, cz.slot, x[j], deg[j]);
+}
 
 // Lane path (degree <= lane_max): up to kc discharge steps.  A step scans the
 // arcs once and pushes along every residual arc to a lower neighbour (the arcs
@@ -1161,6 +1239,11 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) hv4[j] = rv[j] > 0 ? D.height[wv[j]] : INT_MAX;
+      // cascade: the step's pushes are all issued before their old excesses
+      // are read, then the heads that turned active are noted together
+      bool got[4] = {false, false, false, false};
+      long long old4[4] = {0, 0, 0, 0};
+      int d4[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = e0 + j;
@@ -1186,13 +1269,20 @@ __device__ void lane_push_cycles(const Dev& D, int v, int b, int deg, int kc, in
           atomicAdd(&D.rres[e], d);
           atomicAdd(&D.res[p], d);
           atomicSub(&D.rres[p], d);
-          const long long old = add64(&D.excess[w], d);
-          // w turned active: the next cascade sub-round takes it
-          if (cz.slot >= 0) cascade_note(D, cz, old <= 0 && old + d > 0 && !is_term(D, w), w);
+          old4[j] = add64(&D.excess[w], d);
+          d4[j] = d;
+          got[j] = true;
         }
         moved += d;
         t.push += 1;
         t.atom += 5;
+      }
+      if (!PULL && cz.slot >= 0) {
+        // heads that turned active: the next cascade sub-round takes them
+        bool cand[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cand[j] = got[j] && old4[j] <= 0 && old4[j] + d4[j] > 0 && !is_term(D, wv[j]);
+        cascade_note_n<4>(D, cz, cand, wv);
       }
     }
     t.arc += deg;
@@ -1774,6 +1864,7 @@ __global__ void k_relabel_prep(Dev D, Dev* out, int early) {
     c->bcur[0][0] = c->bcur[0][1] = c->bcur[1][0] = c->bcur[1][1] = 0;
     c->early = early;
     c->bfs_cut = 0;
+    c->bfs_rbu = 0;
     if (out) *out = D;
   }
   if (threadIdx.x < kSlots) {
@@ -1845,7 +1936,8 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
     return;
   }
   if (bottomup) {
-    ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+    ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t,
+                    D.nprev > 0 ? D.prev : nullptr, D.nprev, early);
   } else {
     const WList W = wlist(D, lvl % 3, wpk);
     ph_bfs_level(D, r, mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, early);
@@ -1894,8 +1986,9 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
     return;
   }
   Tally t;
-  if (use_bottom_up(D, mode, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
-    ph_bfs_bottomup(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
+  if (use_bottom_up(D, mode, lo, cnt, static_cast<long long>(wpk & 0xffffffffu))) {
+    ph_bfs_bottomup(D, make_range(), lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t,
+                    D.nprev > 0 ? D.prev : nullptr, D.nprev, early);
   } else {
     const WList W = wlist(D, lvl % 3, wpk);
     ph_bfs_level(D, make_range(), mode, lo, cnt, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], W, (lvl + 1) % 3, t, early);
@@ -1952,26 +2045,6 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
   } else {
     ph_push<true>(D, r, D.wl2, cnt, kc, pmode, nullptr, wlist(D, kQPull, wpk), t);
   }
-  t.flush(D.ctrs);
-}
-// One cascade sub-round: pushes from the list in slot `slot` (sizes read from
-// the control block, written by the previous pass) and lists the vertices for
-// the next sub-round in slot (slot+1)%3 with stamp `stamp`.  Block 0 clears
-// slot (slot+2)%3, the input of the sub-round before, for the sub-round
-// after.  An empty list returns at once, so a chain of launches stops doing
-// work when the excess has settled.
-__global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist) {
-  const Ctrl* cc = D.ctrl;
-  const int cnt = cc->qcnt[slot];
-  const unsigned long long wpk = cc->wq[slot];
-  if (is_block0_thread0()) {
-    D.ctrl->qcnt[(slot + 2) % 3] = 0;
-    D.ctrl->wq[(slot + 2) % 3] = 0;
-  }
-  if (cnt == 0 && wpk == 0) return;
-  Tally t;
-  ph_push<false>(D, make_range(), cascade_list(D, slot), cnt, kc, PM_ALL, nullptr, wlist(D, slot, wpk), t,
-                 Cascade{(slot + 1) % 3, stamp, relist});
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, int rmode,
@@ -2076,7 +2149,7 @@ __device__ int relabel_persistent(const Dev& D, cg::grid_group& G, const Range& 
     const int cur = lvl % 3;
     const int cnt = ldr(&D.ctrl->qcnt[cur]);
     if (cnt == 0) break;
-    if (use_bottom_up(D, mode, cnt, static_cast<long long>(ldru64(&D.ctrl->wq[cur]) & 0xffffffffu))) {
+    if (use_bottom_up(D, mode, lo, cnt, static_cast<long long>(ldru64(&D.ctrl->wq[cur]) & 0xffffffffu))) {
       ph_bfs_bottomup(D, r, lvl, &D.ctrl->qcnt[(lvl + 1) % 3], (lvl + 1) % 3, D.order + lo + cnt, t);
     } else {
       const WList W = wlist(D, cur, ldru64(&D.ctrl->wq[cur]));

@@ -49,24 +49,37 @@ struct SchedState {
   int32_t relabel;  // decision: fire the global relabel before the next round
   int32_t variant;  // decision at batch start
   int32_t valid;    // outputs finite
-  int32_t pad;
+  int32_t tag;      // tag of the round whose decision this is (cascade launches)
 };
 
 __device__ __forceinline__ float log2p1(double x) { return static_cast<float>(log2(1.0 + x)); }
 
 // phase 0 = batch start (counters of the batch so far are ignored), 1 = round end.
 // active: vertices listed for the round just run (both lists and both queues),
-// as the host read them before the round's pushes.
-__global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, SchedState* S, int phase,
-                                                 int k, float inc_frac, int rounds_since, int last_levels,
-                                                 double active) {
-  const int lane = threadIdx.x;
-  // totals of the work counters (kSlots copies each)
-  unsigned long long pu = D.ctrs[C_PUSH * kSlots + lane];
-  unsigned long long re = D.ctrs[C_RELABEL * kSlots + lane];
+// as the host read them before the round's pushes.  inc_frac < 0: the
+// fraction of capacity increases is read from the control block (the batch's
+// lookup counted them on the device).  Called by one whole wave
+// (64 lanes).  Every weight a lane needs is loaded before the first product,
+// so the evaluation costs a few memory round trips, not one per hidden unit.
+__device__ void sched_eval(const Dev& D, const SchedWeights* W, SchedState* S, int phase, int k, float inc_frac,
+                           int rounds_since, int last_levels, double active) {
+  const int lane = lane_id();
+  const int hl = lane < kHid ? lane : 0;  // lanes >= kHid compute a copy of unit 0
+  float w1[kFeat], w2[kHid], w3[kHid];
+#pragma unroll
+  for (int i = 0; i < kFeat; ++i) w1[i] = W->w1[i][hl];
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) w2[i] = W->w2[i][hl];
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) w3[i] = W->w3[i][lane < kOut ? lane : 0];
+  const float b1 = W->b1[hl], b2 = W->b2[hl], b3 = W->b3[lane < kOut ? lane : 0];
+  // totals of the work counters (kSlots copies each); uncached, so a caller
+  // inside a cooperative launch sees the other blocks' atomics
+  unsigned long long pu = ldru64(&D.ctrs[C_PUSH * kSlots + lane]);
+  unsigned long long re = ldru64(&D.ctrs[C_RELABEL * kSlots + lane]);
   pu = static_cast<unsigned long long>(wave_sum_i64(static_cast<long long>(pu)));
   re = static_cast<unsigned long long>(wave_sum_i64(static_cast<long long>(re)));
-  const long long sink = D.excess[D.t];
+  const long long sink = ldr64(&D.excess[D.t]);
   float f[kFeat];
   const float ln = log2p1(D.n), lm = log2p1(D.m);
   if (phase == 0) {
@@ -85,26 +98,19 @@ __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, S
     f[7] = log2p1(last_levels) / 8.f;
   }
   f[4] = log2p1(k) / lm;
-  f[5] = inc_frac;
+  f[5] = inc_frac >= 0.f ? inc_frac : (k > 0 ? static_cast<float>(ldr(&D.ctrl->inc)) / static_cast<float>(k) : 0.5f);
   // layer 1: lane j < kHid computes hidden unit j
-  float h1 = 0.f;
-  if (lane < kHid) {
-    h1 = W->b1[lane];
+  float h1 = b1;
 #pragma unroll
-    for (int i = 0; i < kFeat; ++i) h1 += f[i] * W->w1[i][lane];
-    h1 = h1 > 0.f ? h1 : 0.f;
-  }
-  float h2 = lane < kHid ? W->b2[lane] : 0.f;
-  for (int i = 0; i < kHid; ++i) {
-    const float hi = __shfl(h1, i);
-    if (lane < kHid) h2 += hi * W->w2[i][lane];
-  }
-  h2 = (lane < kHid && h2 > 0.f) ? h2 : 0.f;
-  float o = lane < kOut ? W->b3[lane] : 0.f;
-  for (int i = 0; i < kHid; ++i) {
-    const float hi = __shfl(h2, i);
-    if (lane < kOut) o += hi * W->w3[i][lane];
-  }
+  for (int i = 0; i < kFeat; ++i) h1 += f[i] * w1[i];
+  h1 = h1 > 0.f ? h1 : 0.f;
+  float h2 = b2;
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) h2 += __shfl(h1, i) * w2[i];
+  h2 = h2 > 0.f ? h2 : 0.f;
+  float o = b3;
+#pragma unroll
+  for (int i = 0; i < kHid; ++i) o += __shfl(h2, i) * w3[i];
   float outs[kOut];
 #pragma unroll
   for (int j = 0; j < kOut; ++j) outs[j] = __shfl(o, j);
@@ -123,6 +129,54 @@ __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, S
     S->prev_relabel = re;
     S->prev_sink = sink;
   }
+}
+
+__global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, SchedState* S, int phase,
+                                                 int k, float inc_frac, int rounds_since, int last_levels,
+                                                 double active) {
+  sched_eval(D, W, S, phase, k, inc_frac, rounds_since, last_levels, active);
+}
+
+// Scheduler inputs of a round-end decision taken inside a cascade launch.
+struct SchedRound {
+  int k;
+  float inc_frac;
+  int rounds_since;
+  int last_levels;
+  double active;
+  int32_t tag;  // the round's tag: the decision is taken once per tag
+};
+
+// One cascade sub-round: pushes from the list in slot `slot` (sizes read from
+// the control block, written by the previous pass) and lists the vertices for
+// the next sub-round in slot (slot+1)%3 with stamp `stamp`.  Block 0 clears
+// slot (slot+2)%3, the input of the sub-round before, for the sub-round
+// after.  An empty list returns at once, so a chain of launches stops doing
+// work when the excess has settled.  W != nullptr: the first launch of the
+// round that finds its list empty, i.e. the launch right after the chain's
+// last working sub-round, also takes the scheduler's round-end decision
+// (wave 0 of block 0; every earlier launch of the stream has completed, so
+// the round's counters are final) and marks it with the round's tag.
+__global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist,
+                                                    const SchedWeights* W, SchedState* S, SchedRound a) {
+  const Ctrl* cc = D.ctrl;
+  const int cnt = cc->qcnt[slot];
+  const unsigned long long wpk = cc->wq[slot];
+  if (is_block0_thread0()) {
+    D.ctrl->qcnt[(slot + 2) % 3] = 0;
+    D.ctrl->wq[(slot + 2) % 3] = 0;
+  }
+  if (cnt == 0 && wpk == 0) {
+    if (W != nullptr && blockIdx.x == 0 && threadIdx.x < kWave && ldr(&S->tag) != a.tag) {
+      sched_eval(D, W, S, 1, a.k, a.inc_frac, a.rounds_since, a.last_levels, a.active);
+      if (threadIdx.x == 0) str(&S->tag, a.tag);
+    }
+    return;
+  }
+  Tally t;
+  ph_push<false>(D, make_range(), cascade_list(D, slot), cnt, kc, PM_ALL, nullptr, wlist(D, slot, wpk), t,
+                 Cascade{(slot + 1) % 3, stamp, relist});
+  t.flush(D.ctrs);
 }
 
 }  // namespace dmfdev

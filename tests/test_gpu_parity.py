@@ -553,6 +553,42 @@ def test_cascade_subrounds_keep_results(gpu, mode, variant, monkeypatch):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("env", [("DMF_BFS_RESTRICT", "1"), ("DMF_BFS_RESTRICT", "64"),
+                                 ("DMF_BFS_BOTTOM_UP", "100000")])
+def test_bottom_up_relabels_keep_results(gpu, mode, env, monkeypatch):
+    """Bottom-up BFS levels.  DMF_BFS_RESTRICT: the relabels from t inside a
+    solve try only the last complete labelled set as candidates, at nearly every
+    level (1) or only at the largest frontiers (64), under the cascade schedule
+    and the learned scheduler.  DMF_BFS_BOTTOM_UP: every relabel from t takes
+    its levels bottom-up over all vertices.  Exact after every batch on an
+    R-MAT graph and on a grid, including capacity increases that let more
+    vertices reach t."""
+    dmf = gpu
+    if env[0] == "DMF_BFS_BOTTOM_UP":
+        monkeypatch.setenv("DMF_BFS_RESTRICT", "0")
+    monkeypatch.setenv(*env)
+    cases = [rmat_case(12, 16, 16)]
+    u, v, w = dmf.gen_grid(32, 32, 6, 1, 1000)
+    gg = dmf.build_csr(u, v, w)
+    cases.append((gg, 0, gg.n - 1, (u, v, w)))
+    for g, s, t, (u, v, w) in cases:
+        say(f"[bottom-up] {env[0]}={env[1]} n={g.n} mode={mode}")
+        with dmf.Engine(g, s, t) as eng:
+            p0 = dmf.default_params(g, mode=mode, cascade=64)
+            f = eng.static_solve(p0)
+            check_engine(eng, g, g.cap, s, t, f, "static")
+            cur = w.copy()
+            applied = []
+            for b, pct in enumerate((2, 6, 1, 4)):
+                hi = 1000 if g is gg else 100
+                applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1600 + b, cap_hi=hi))
+                p = (dmf.default_params(g, mode=mode, cascade=64) if b % 2 == 0
+                     else dmf.default_params(g, mode=mode, scheduler=dmf.SCHED_LEARNED))
+                f = eng.apply_batch(*applied[-1], p)
+                check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+@pytest.mark.parametrize("mode", [1, 2])
 def test_cascade_parameter_on_rmat_and_grid(gpu, mode):
     """The cascade parameter (sub-rounds chained on the device, launched in
     checked chunks) under the fixed schedule and under the learned scheduler

@@ -2,8 +2,9 @@
 //
 //   dmf_multi_bench [--instances=256] [--log-lo=10] [--log-hi=16] [--ef-lo=8]
 //                   [--ef-hi=32] [--seed=4] [--batches=5] [--pct=5]
-//                   [--repeats=3] [--mode=persistent|stepped|graph|all]
-//                   [--variant=data|topo|pp] [--no-autotune] [--no-solo] [--json]
+//                   [--repeats=3] [--mode=persistent|stepped|graph|all, or a comma list]
+//                   [--variant=data|topo|pp] [--sched=plain|learned]
+//                   [--cascade=K] [--no-autotune] [--no-solo] [--json]
 //
 // Workload (all seeded): instance i draws x uniform in [log_lo, log_hi) and
 // has n_i = floor(2^x) vertices, an edge factor uniform in [ef_lo, ef_hi] and
@@ -44,6 +45,8 @@ struct Opts {
   int repeats = 3;
   std::string mode = "all";
   int variant = DMF_VARIANT_DATA;
+  int sched = DMF_SCHED_FIXED;  // of the batched engine (the solo baseline keeps the fixed schedule)
+  int cascade = 0;              // cascade sub-rounds per round of the batched engine
   bool autotune = true;
   bool solo = true;
   bool json = false;
@@ -151,6 +154,10 @@ int main(int argc, char** argv) {
     else if ((a = val("--variant="))) {
       o.variant = !std::strcmp(a, "topo") ? DMF_VARIANT_TOPO : !std::strcmp(a, "pp") ? DMF_VARIANT_PP
                                                                                        : DMF_VARIANT_DATA;
+    } else if ((a = val("--sched="))) {
+      o.sched = !std::strcmp(a, "learned") ? DMF_SCHED_LEARNED : DMF_SCHED_FIXED;
+    } else if ((a = val("--cascade="))) {
+      o.cascade = std::atoi(a);
     } else if (!std::strcmp(x, "--no-autotune")) o.autotune = false;
     else if (!std::strcmp(x, "--no-solo")) o.solo = false;
     else if (!std::strcmp(x, "--json")) o.json = true;
@@ -267,8 +274,20 @@ int main(int argc, char** argv) {
   if (rc) die("batch_create", rc);
   std::printf("batched engine: %lld device bytes\n", static_cast<long long>(dmf_engine_device_bytes(E)));
   std::vector<std::string> modes;
-  if (o.mode == "all") modes = {"persistent", "stepped", "graph"};
-  else modes = {o.mode};
+  if (o.mode == "all") {
+    modes = {"persistent", "stepped", "graph"};
+  } else {
+    for (size_t a = 0; a <= o.mode.size();) {  // comma-separated list
+      const size_t b = std::min(o.mode.find(',', a), o.mode.size());
+      if (b > a) modes.push_back(o.mode.substr(a, b - a));
+      a = b + 1;
+    }
+  }
+  for (const std::string& mname : modes)
+    if (mname != "persistent" && mname != "stepped" && mname != "graph") {
+      std::fprintf(stderr, "dmf_multi_bench: unknown mode %s\n", mname.c_str());
+      return 2;
+    }
   std::vector<ModeResult> results;
   int mismatches = 0;
   std::vector<int64_t> flows(o.instances);
@@ -277,6 +296,8 @@ int main(int argc, char** argv) {
     R.name = mname;
     dmf_params q = p;
     q.mode = mname == "persistent" ? 0 : mname == "stepped" ? 1 : 2;
+    q.scheduler = o.sched;
+    q.cascade = o.cascade;
     if (o.autotune) {
       if ((rc = dmf_engine_autotune(E, &q, &R.lane_max, &R.wide_t))) die("autotune", rc);
     }
@@ -328,10 +349,11 @@ int main(int argc, char** argv) {
   if (o.solo) solo_p50 = pct_rank(solo_batch_ms, 50);
   if (o.json) {
     std::printf("{\"workload\":\"%d RMAT instances, n log-uniform 2^%g-2^%g, EF %d-%d, seed %llu, %d batches of %g%%\","
-                "\"vertices\":%lld,\"arcs\":%lld,\"updates\":%lld,\"solo_batch_p50_ms\":%.4f,\"verified\":%s,"
+                "\"scheduler\":\"%s\",\"cascade\":%d,\"vertices\":%lld,\"arcs\":%lld,\"updates\":%lld,\"solo_batch_p50_ms\":%.4f,\"verified\":%s,"
                 "\"mismatches\":%d,\"modes\":[",
                 o.instances, o.log_lo, o.log_hi, o.ef_lo, o.ef_hi, static_cast<unsigned long long>(o.seed),
-                o.batches, o.pct, static_cast<long long>(n_tot), static_cast<long long>(m_tot),
+                o.batches, o.pct, o.sched == DMF_SCHED_LEARNED ? "learned" : "plain", o.cascade,
+                static_cast<long long>(n_tot), static_cast<long long>(m_tot),
                 static_cast<long long>(upd_tot), solo_p50, o.solo ? "true" : "false", mismatches);
     for (size_t r = 0; r < results.size(); ++r) {
       const ModeResult& R = results[r];

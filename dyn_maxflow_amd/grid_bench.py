@@ -1,17 +1,21 @@
 """Learned scheduler vs fixed-frequency global relabel on a high-diameter grid.
 
   python -m dyn_maxflow_amd.grid_bench [--side 4096] [--seed 3] [--pct 10]
-         [--schedules learned,fixed16] [--mode 2] [--static-interval 16]
+         [--schedules learned,fixed1,fixed1c1024] [--mode 2]
+         [--static-interval 1] [--static-cascade 1024]
          [--progress FILE] [--no-check]
 
 Workload: side x side 4-neighbour grid (every neighbour pair in both
 directions, independent capacities uniform [1,1000]), s = corner (0,0),
 t = corner (side-1, side-1), seed 3; one batch updating pct% of the input
 arcs (half increases, half decreases).  For every schedule a fresh engine is
-built, solved statically (fixed interval --static-interval, the same for all
-schedules), and then takes the batch under that schedule:
+built, solved statically (data variant, global relabel every
+--static-interval rounds, at most --static-cascade push sub-rounds after
+each round; the same for all schedules), and then takes the batch under
+that schedule:
   learned   the device MLP decides when to relabel (and the variant);
-  fixedN    data variant, global relabel every N rounds (fixed1 = reference).
+  fixedN    data variant, global relabel every N rounds (fixed1 = reference);
+  fixedNcK  the same with at most K cascade push sub-rounds after each round.
 The batch is timed with HIP events on the engine's stream.  Checks: every
 schedule reports the same flow, and a from-scratch static solve of the
 updated graph (the reference's dynamic == static check) reports it too.
@@ -51,9 +55,10 @@ def main():
     ap.add_argument("--side", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--pct", type=float, default=10.0)
-    ap.add_argument("--schedules", default="learned,fixed16")
+    ap.add_argument("--schedules", default="learned,fixed1,fixed1c1024")
     ap.add_argument("--mode", type=int, default=2, help="driver: 1 stepped, 2 graph-replayed BFS levels")
-    ap.add_argument("--static-interval", type=int, default=16)
+    ap.add_argument("--static-interval", type=int, default=1)
+    ap.add_argument("--static-cascade", type=int, default=1024)
     ap.add_argument("--progress", default=None)
     ap.add_argument("--no-check", action="store_true")
     a = ap.parse_args()
@@ -70,13 +75,14 @@ def main():
     print(f"[grid] {a.side}x{a.side}: n={g.n} m={g.m} batch={len(src)} ({time.time() - t0:.1f}s)", flush=True)
     if a.progress:
         os.environ["DMF_TRACE"] = a.progress
-    static_p = dmf.default_params(g, mode=a.mode, relabel_interval=a.static_interval)
+    static_p = dmf.default_params(g, mode=a.mode, relabel_interval=a.static_interval, cascade=a.static_cascade)
     rows = []
     for name in a.schedules.split(","):
         if name == "learned":
             p = dmf.default_params(g, mode=a.mode, scheduler=dmf.SCHED_LEARNED)
         elif name.startswith("fixed"):
-            p = dmf.default_params(g, mode=a.mode, relabel_interval=int(name[5:]))
+            iv, _, casc = name[5:].partition("c")
+            p = dmf.default_params(g, mode=a.mode, relabel_interval=int(iv), cascade=int(casc or 0))
         else:
             raise SystemExit(f"unknown schedule {name}")
         with dmf.Engine(g, s, t) as eng:
@@ -105,7 +111,7 @@ def main():
         print(f"[grid] static recompute of the updated grid: flow {check}", flush=True)
     out = {"workload": f"grid {a.side}x{a.side} seed {a.seed}, caps U[1,1000], one batch of {a.pct}% of arcs",
            "n": g.n, "m": g.m, "k": int(len(src)), "driver": {1: "stepped", 2: "graph-replayed BFS"}.get(a.mode),
-           "static_interval": a.static_interval, "schedules": rows,
+           "static_interval": a.static_interval, "static_cascade": a.static_cascade, "schedules": rows,
            "flows_agree": len(flows) == 1 and (check is None or check in flows), "static_recompute_flow": check}
     base = [r for r in rows if r["schedule"] != "learned"]
     learned = [r for r in rows if r["schedule"] == "learned"]

@@ -183,7 +183,7 @@ def test_grid4096_one_batch_certified(gpu):
     cur = w.copy()
     batch = dmf.gen_updates(u, v, cur, int(len(u) * 10 / 100), 3 * 1000 + 1, inc_frac=0.5, cap_lo=1, cap_hi=1000)
     cap1 = caps_sequence(g, [batch])[0]
-    static_p = dmf.default_params(g, relabel_interval=16)
+    static_p = dmf.default_params(g, cascade=1024)
     say(f"[grid] n={g.n} m={g.m} k={len(batch[0])}: static solve")
     with dmf.Engine(g, s, t) as eng:
         f0 = eng.static_solve(static_p)

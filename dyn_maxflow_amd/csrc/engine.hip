@@ -523,7 +523,10 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       bool sched_done = false;  // the cascade chain took the round-end decision
       if (!topo) {
         // the list counters were cleared by the last push launch (or k_begin)
-        LAUNCH(k_build_wl, blocks_for(visited), D, visited);
+        // a labelled set that is a large part of the graph is listed by one
+        // coalesced pass over every vertex instead of a gather through order[]
+        const bool all = visited > n / 4;
+        LAUNCH(k_build_wl, blocks_for(all ? n : visited), D, all ? -1 : visited);
         if ((rc = sync_ctrl(E))) return rc;
         cnt = E->h_ctrl->wl_cnt;
         wpk = E->h_ctrl->wq[kQActive];

@@ -617,8 +617,17 @@ __device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1]
 // ctrl->qcnt[0] and ctrl->wq[0] must be 0.
 // early (BFS_SINK only): also mark as targets the vertices that are active
 // now (not a terminal, excess > 0, height < n before this relabel) and count
-// them.
+// them.  Roots are staged per wave in LDS and published per block (wide ones
+// through the block's wide stage): thousands of deficit roots appended one
+// atomic per wave serialised on the frontier counter.  Every thread of the
+// block must call it.
 __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t, bool early = false) {
+  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
+  Stage st{sbuf[threadIdx.x / kWave], 0};
+  const WideStage WS{wvert, wchunks, &wcnt};
+  if (threadIdx.x == 0) wcnt = 0;
+  __syncthreads();
   const int n = D.n;
   long long targets = 0;
   for (int base = r.gid - lane_id(); base < n; base += r.gsz) {
@@ -646,19 +655,16 @@ __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t, bo
       D.height[v] = root ? 0 : n;
       D.lev16[v] = root ? 0 : kLevFree;
       t.vert += 1;
-      if (root) {
-        const int d = degree(D, v);
-        if (d > D.wide_t) wq_append(D, 0, v, d);
-      }
     }
-    int rank;
-    const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
-    if (root) D.order[off + rank] = v;
+    wide_note(D, WS, 0, root && degree(D, v) > D.wide_t, v);
+    st.push(root, v, &D.ctrl->qcnt[0], D.order);
   }
   if (early) {
     targets = wave_sum_i64(targets);
     if (lane_id() == 0 && targets) atomicAdd(&D.ctrs[C_TGT * kSlots + (r.wid % kSlots)], targets);
   }
+  st.flush_block(&D.ctrl->qcnt[0], D.order);  // its barriers also publish the wide stage
+  wide_flush(D, WS, 0);
 }
 
 // Whether every target of the current early-exit relabel is labelled.  All 64
@@ -999,25 +1005,38 @@ __host__ __device__ __forceinline__ bool use_bottom_up(const Dev& D, int mode, i
 // local relabels), a larger one chunk by chunk.  Those up to lane_max are
 // listed for one lane each.
 __device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
-  for (int base = r.gid - lane_id(); base < cnt; base += r.gsz) {
+  // cnt < 0: every vertex in id order, one coalesced pass (for a labelled
+  // set that is a large part of the graph).  The set listed is the same: a
+  // vertex below height n is either labelled by the last relabel or was
+  // given a label by k_cap_unlabelled, which only labels vertices without
+  // excess (or, after a lazy start, has since received excess downhill and is
+  // active with a valid label).  Appends are staged in LDS and published per
+  // block; every thread of the block must call it.
+  __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
+  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
+  Stage st{sbuf[threadIdx.x / kWave], 0};
+  const WideStage WS{wvert, wchunks, &wcnt};
+  if (threadIdx.x == 0) wcnt = 0;
+  __syncthreads();
+  const int total = cnt < 0 ? D.n : cnt;
+  for (int base = r.gid - lane_id(); base < total; base += r.gsz) {
     const int i = base + lane_id();
     bool act = false, wide = false;
-    int v = -1, d = 0;
-    if (i < cnt) {
-      v = D.order[i];
+    int v = -1;
+    if (i < total) {
+      v = cnt < 0 ? i : D.order[i];
       act = !is_term(D, v) && D.excess[v] > 0 && D.height[v] < D.n;
       t.vert += 1;
       if (act) {
-        d = degree(D, v);
-        wide = d > D.lane_max;
+        wide = degree(D, v) > D.lane_max;
         act = !wide;
       }
     }
-    wave_wq_append(D, kQActive, wide, v, d);
-    int rank;
-    const int off = wave_reserve(&D.ctrl->wl_cnt, act, &rank);
-    if (act) D.wl[off + rank] = v;
+    wide_note(D, WS, kQActive, wide, v);
+    st.push(act, v, &D.ctrl->wl_cnt, D.wl);
   }
+  st.flush_block(&D.ctrl->wl_cnt, D.wl);  // its barriers also publish the wide stage
+  wide_flush(D, WS, kQActive);
 }
 
 // create_pp_wl (code/dyn_pp.h:499-545): B vertices with excess > 0 to wl,

@@ -59,20 +59,27 @@ __device__ __forceinline__ float log2p1(double x) { return static_cast<float>(lo
 // as the host read them before the round's pushes.  inc_frac < 0: the
 // fraction of capacity increases is read from the control block (the batch's
 // lookup counted them on the device).  Called by one whole wave
-// (64 lanes).  Every weight a lane needs is loaded before the first product,
-// so the evaluation costs a few memory round trips, not one per hidden unit.
+// (64 lanes).  HOIST (weights in global memory): every weight a lane needs is
+// loaded before the first product, so the evaluation costs a few memory
+// round trips, not one per hidden unit, at the price of about 115 registers.
+// Without it (weights already in LDS) the products read them as they go,
+// which keeps the caller's register count, and its occupancy, low.
+template <bool HOIST>
 __device__ void sched_eval(const Dev& D, const SchedWeights* W, SchedState* S, int phase, int k, float inc_frac,
                            int rounds_since, int last_levels, double active) {
   const int lane = lane_id();
   const int hl = lane < kHid ? lane : 0;  // lanes >= kHid compute a copy of unit 0
-  float w1[kFeat], w2[kHid], w3[kHid];
+  const int ol = lane < kOut ? lane : 0;
+  float w1[kFeat], w2[HOIST ? kHid : 1], w3[HOIST ? kHid : 1];
 #pragma unroll
   for (int i = 0; i < kFeat; ++i) w1[i] = W->w1[i][hl];
+  if (HOIST) {
 #pragma unroll
-  for (int i = 0; i < kHid; ++i) w2[i] = W->w2[i][hl];
+    for (int i = 0; i < kHid; ++i) w2[i] = W->w2[i][hl];
 #pragma unroll
-  for (int i = 0; i < kHid; ++i) w3[i] = W->w3[i][lane < kOut ? lane : 0];
-  const float b1 = W->b1[hl], b2 = W->b2[hl], b3 = W->b3[lane < kOut ? lane : 0];
+    for (int i = 0; i < kHid; ++i) w3[i] = W->w3[i][ol];
+  }
+  const float b1 = W->b1[hl], b2 = W->b2[hl], b3 = W->b3[ol];
   // totals of the work counters (kSlots copies each); uncached, so a caller
   // inside a cooperative launch sees the other blocks' atomics
   unsigned long long pu = ldru64(&D.ctrs[C_PUSH * kSlots + lane]);
@@ -105,12 +112,12 @@ __device__ void sched_eval(const Dev& D, const SchedWeights* W, SchedState* S, i
   for (int i = 0; i < kFeat; ++i) h1 += f[i] * w1[i];
   h1 = h1 > 0.f ? h1 : 0.f;
   float h2 = b2;
-#pragma unroll
-  for (int i = 0; i < kHid; ++i) h2 += __shfl(h1, i) * w2[i];
+#pragma unroll 8
+  for (int i = 0; i < kHid; ++i) h2 += __shfl(h1, i) * (HOIST ? w2[HOIST ? i : 0] : W->w2[i][hl]);
   h2 = h2 > 0.f ? h2 : 0.f;
   float o = b3;
-#pragma unroll
-  for (int i = 0; i < kHid; ++i) o += __shfl(h2, i) * w3[i];
+#pragma unroll 8
+  for (int i = 0; i < kHid; ++i) o += __shfl(h2, i) * (HOIST ? w3[HOIST ? i : 0] : W->w3[i][ol]);
   float outs[kOut];
 #pragma unroll
   for (int j = 0; j < kOut; ++j) outs[j] = __shfl(o, j);
@@ -134,7 +141,7 @@ __device__ void sched_eval(const Dev& D, const SchedWeights* W, SchedState* S, i
 __global__ void __launch_bounds__(kWave) k_sched(Dev D, const SchedWeights* W, SchedState* S, int phase,
                                                  int k, float inc_frac, int rounds_since, int last_levels,
                                                  double active) {
-  sched_eval(D, W, S, phase, k, inc_frac, rounds_since, last_levels, active);
+  sched_eval<true>(D, W, S, phase, k, inc_frac, rounds_since, last_levels, active);
 }
 
 // Scheduler inputs of a round-end decision taken inside a cascade launch.
@@ -155,8 +162,9 @@ struct SchedRound {
 // work when the excess has settled.  W != nullptr: the first launch of the
 // round that finds its list empty, i.e. the launch right after the chain's
 // last working sub-round, also takes the scheduler's round-end decision
-// (wave 0 of block 0; every earlier launch of the stream has completed, so
-// the round's counters are final) and marks it with the round's tag.
+// (block 0 stages the weights in LDS, wave 0 evaluates; every earlier launch
+// of the stream has completed, so the round's counters are final) and marks
+// it with the round's tag.
 __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist,
                                                     const SchedWeights* W, SchedState* S, SchedRound a) {
   const Ctrl* cc = D.ctrl;
@@ -167,9 +175,17 @@ __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int
     D.ctrl->wq[(slot + 2) % 3] = 0;
   }
   if (cnt == 0 && wpk == 0) {
-    if (W != nullptr && blockIdx.x == 0 && threadIdx.x < kWave && ldr(&S->tag) != a.tag) {
-      sched_eval(D, W, S, 1, a.k, a.inc_frac, a.rounds_since, a.last_levels, a.active);
-      if (threadIdx.x == 0) str(&S->tag, a.tag);
+    // uniform over block 0: the tag was written by an earlier launch
+    if (W != nullptr && blockIdx.x == 0 && ldr(&S->tag) != a.tag) {
+      __shared__ SchedWeights wl;
+      const float* src = reinterpret_cast<const float*>(W);
+      float* dst = reinterpret_cast<float*>(&wl);
+      for (int i = threadIdx.x; i < kSchedParams; i += blockDim.x) dst[i] = src[i];
+      __syncthreads();
+      if (threadIdx.x < kWave) {
+        sched_eval<false>(D, &wl, S, 1, a.k, a.inc_frac, a.rounds_since, a.last_levels, a.active);
+        if (threadIdx.x == 0) str(&S->tag, a.tag);
+      }
     }
     return;
   }

@@ -1211,6 +1211,29 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
     dmf_engine_destroy(E);
     return DMF_EDEVICE;
   }
+  if (inst && ninst > 1) {
+    // chunk bases of the terminals' arcs: the saturation of many sources (or
+    // sinks) spreads their arcs over the grid
+    std::vector<int32_t> cb;
+    auto upload = [&](const std::vector<int32_t>& T, const int32_t** out, int32_t* chunks) -> bool {
+      cb.assign(T.size(), 0);
+      int64_t run = 0;
+      for (size_t i = 0; i < T.size(); ++i) {
+        cb[i] = static_cast<int32_t>(run);
+        const int64_t d = static_cast<int64_t>(g->row[T[i] + 1]) - g->row[T[i]];
+        run += (d + kChunk - 1) / kChunk;
+      }
+      int32_t* dev = nullptr;
+      if (run > INT_MAX || dalloc(E, &dev, T.size()) != DMF_OK) return false;
+      *out = dev;
+      *chunks = static_cast<int32_t>(run);
+      return hipMemcpy(dev, cb.data(), sizeof(int32_t) * T.size(), hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!upload(srcs, &D.src_cbase, &D.src_chunks) || !upload(snks, &D.snk_cbase, &D.snk_chunks)) {
+      dmf_engine_destroy(E);
+      return DMF_EDEVICE;
+    }
+  }
   hipLaunchKernelGGL(k_build_hash, dim3(blocks_for(m)), dim3(kBlock), 0, E->stream, D);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E->stream) != hipSuccess) {
     dmf_engine_destroy(E);
@@ -1476,6 +1499,8 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(const_cast<uint8_t*>(D.term));
     hipFree(const_cast<int32_t*>(D.srcs));
     hipFree(const_cast<int32_t*>(D.snks));
+    hipFree(const_cast<int32_t*>(D.src_cbase));
+    hipFree(const_cast<int32_t*>(D.snk_cbase));
     hipFree(const_cast<int32_t*>(D.inst));
     hipFree(D.iflow);
     hipFree(D.lev16);

@@ -96,6 +96,14 @@ struct Dev {
   int32_t nsrc;
   const int32_t* snks;
   int32_t nsnk;
+  // With more than one source (sink): chunk c of kChunk arcs belongs to
+  // terminal i when src_cbase[i] <= c < src_cbase[i+1] (the first nsrc
+  // entries are given; src_chunks is the total), so saturation spreads every
+  // terminal's arcs over the grid instead of looping over the terminals.
+  const int32_t* src_cbase;
+  int32_t src_chunks;
+  const int32_t* snk_cbase;
+  int32_t snk_chunks;
   const int32_t* inst;  // instance of every vertex (nullptr: a single instance)
   long long* iflow;     // per-instance flow
   int32_t ninst;
@@ -563,7 +571,36 @@ __device__ void saturate_one_source(const Dev& D, const Range& r, int s, Tally& 
 }
 
 __device__ void ph_saturate_source(const Dev& D, const Range& r, Tally& t) {
-  for (int si = 0; si < D.nsrc; ++si) saturate_one_source(D, r, D.srcs[si], t);
+  if (D.nsrc == 1 || !D.src_cbase) {
+    for (int si = 0; si < D.nsrc; ++si) saturate_one_source(D, r, D.srcs[si], t);
+    return;
+  }
+  // many sources (a batch of instances): one chunk of one source per wave
+  for (int c = r.wid; c < D.src_chunks; c += r.nw) {
+    const int si = wave_find_slot(D.src_cbase, D.nsrc, c);
+    const int s = D.srcs[si];
+    const int b = D.row[s], deg = D.row[s + 1] - b;
+    const int j0 = (c - D.src_cbase[si]) * kChunk;
+    const int j1 = min(deg, j0 + kChunk);
+    long long sent = 0;
+    for (int j = j0 + lane_id(); j < j1; j += kWave) {
+      const int e = b + j;
+      const int d = D.res[e];
+      if (d > 0) {
+        add64(&D.excess[D.col[e]], d);
+        D.res[e] = 0;
+        D.rres[e] += d;
+        const int p = D.rev[e];
+        D.res[p] += d;
+        D.rres[p] -= d;
+        sent += d;
+        t.atom += 1;
+      }
+      t.arc += 1;
+    }
+    sent = wave_sum_i64(sent);
+    if (lane_id() == 0 && sent) add64(&D.excess[s], -sent);
+  }
 }
 // Second half of the push-pull static saturation (code/dyn_pp.h:56-70): pull
 // every residual arc into t.
@@ -594,7 +631,35 @@ __device__ void saturate_one_sink_pp(const Dev& D, const Range& r, int v, Tally&
 }
 
 __device__ void ph_saturate_sink_pp(const Dev& D, const Range& r, Tally& t) {
-  for (int si = 0; si < D.nsnk; ++si) saturate_one_sink_pp(D, r, D.snks[si], t);
+  if (D.nsnk == 1 || !D.snk_cbase) {
+    for (int si = 0; si < D.nsnk; ++si) saturate_one_sink_pp(D, r, D.snks[si], t);
+    return;
+  }
+  for (int c = r.wid; c < D.snk_chunks; c += r.nw) {
+    const int si = wave_find_slot(D.snk_cbase, D.nsnk, c);
+    const int v = D.snks[si];
+    const int b = D.row[v], deg = D.row[v + 1] - b;
+    const int j0 = (c - D.snk_cbase[si]) * kChunk;
+    const int j1 = min(deg, j0 + kChunk);
+    long long got = 0;
+    for (int j = j0 + lane_id(); j < j1; j += kWave) {
+      const int e = b + j;
+      const int d = D.rres[e];
+      if (d > 0) {
+        add64(&D.excess[D.col[e]], -static_cast<long long>(d));
+        D.res[e] += d;
+        D.rres[e] -= d;
+        const int p = D.rev[e];
+        D.res[p] -= d;
+        D.rres[p] += d;
+        got += d;
+        t.atom += 1;
+      }
+      t.arc += 1;
+    }
+    got = wave_sum_i64(got);
+    if (lane_id() == 0 && got) add64(&D.excess[v], got);
+  }
 }
 
 // ----------------------------------------------------------- global relabel

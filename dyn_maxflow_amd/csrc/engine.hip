@@ -888,6 +888,8 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   int32_t *wq_vert = nullptr, *wq_base = nullptr, *wide_vert = nullptr, *wide_base = nullptr;
   int32_t *wave_vert = nullptr, *wave_base = nullptr;
   uint32_t* widebits = nullptr;
+  int4* bfs_ct = nullptr;
+  const size_t ct_cap = static_cast<size_t>(std::max<int64_t>(1, wide_chunks));
   const int64_t before = E->bytes;
   auto put = [](void* dst, const void* src, size_t bytes) {
     return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -896,7 +898,7 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   if (!(rc = dalloc(E, &wq_vert, nqueue * kNumQ)) && !(rc = dalloc(E, &wq_base, nqueue * kNumQ)) &&
       !(rc = dalloc(E, &wide_vert, nwide)) && !(rc = dalloc(E, &wide_base, nwide)) &&
       !(rc = dalloc(E, &wave_vert, nqueue)) && !(rc = dalloc(E, &wave_base, nqueue)) &&
-      !(rc = dalloc(E, &widebits, wbits.size()))) {
+      !(rc = dalloc(E, &widebits, wbits.size())) && !(rc = dalloc(E, &bfs_ct, ct_cap * kNumRing))) {
     const bool copied = put(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size()) &&
                         put(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size()) &&
                         put(wave_vert, wave_v.data(), sizeof(int32_t) * wave_v.size()) &&
@@ -907,7 +909,7 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   if (rc) {
     for (void* p : {static_cast<void*>(wq_vert), static_cast<void*>(wq_base), static_cast<void*>(wide_vert),
                     static_cast<void*>(wide_base), static_cast<void*>(wave_vert),
-                    static_cast<void*>(wave_base), static_cast<void*>(widebits)})
+                    static_cast<void*>(wave_base), static_cast<void*>(widebits), static_cast<void*>(bfs_ct)})
       if (p) (void)hipFree(p);
     E->bytes = before;
     return rc;
@@ -917,7 +919,7 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
                   const_cast<void*>(static_cast<const void*>(D.wide_base)),
                   const_cast<void*>(static_cast<const void*>(D.wave_vert)),
                   const_cast<void*>(static_cast<const void*>(D.wave_base)),
-                  const_cast<void*>(static_cast<const void*>(D.widebits))})
+                  const_cast<void*>(static_cast<const void*>(D.widebits)), static_cast<void*>(D.bfs_ct)})
     if (p) (void)hipFree(p);
   const int64_t new_bytes = E->bytes - before;
   E->bytes -= E->wide_bytes;
@@ -931,6 +933,8 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   D.wide_vert = wide_vert;
   D.wide_base = wide_base;
   D.widebits = widebits;
+  D.bfs_ct = bfs_ct;
+  D.bfs_ct_cap = static_cast<int32_t>(ct_cap);
   D.wide_t = wide_t;
   D.wide_packed = (static_cast<unsigned long long>(wide_v.size()) << 32) |
                   static_cast<unsigned long long>(wide_chunks);
@@ -1507,6 +1511,7 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(D.tgt);
     hipFree(D.cmark);
     hipFree(const_cast<uint32_t*>(D.widebits));
+    hipFree(D.bfs_ct);
     hipFree(D.wq_vert);
     hipFree(D.wq_base);
     hipFree(const_cast<int32_t*>(D.wide_vert));
@@ -1738,6 +1743,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)
+  b += 3 * 16 * (M / kChunk + M / 64 + 1);      // BFS ring chunk tables: chunks of the vertices above wide_t
   b += 48 * std::min(N + 1, M / 17 + 1);        // queues and wave list: per vertex above lane_max (>= 16)
   b += static_cast<int64_t>(C_N) * kSlots * 8;  // counters
   b += int64_t{1} << 20;                        // control block, scheduler, allocation slack

@@ -61,6 +61,7 @@ enum CtrIdx {
 };
 
 // Wide queues: 0..2 BFS frontier ring (level % 3), 3 active list, 4 pull list.
+constexpr int kNumRing = 3;
 constexpr int kQActive = 3;
 constexpr int kQPull = 4;
 constexpr int kNumQ = 5;
@@ -152,6 +153,14 @@ struct Dev {
   const int32_t* wave_base;
   unsigned long long wave_packed;
   const uint32_t* widebits;  // bit v set when degree(v) > wide_t
+  // Chunk tables of the BFS ring queues (kNumRing x bfs_ct_cap entries, null
+  // when not allocated): entry c of ring queue q is chunk c of that queue as
+  // {vertex, first arc, end arc, 0}, written when the vertex is appended, so
+  // a level finds a chunk's arcs with one load instead of a search over the
+  // queue's slots.  A BFS appends every vertex at most once per ring slot,
+  // so bfs_ct_cap = the chunks of all vertices above wide_t.
+  int4* bfs_ct;
+  int32_t bfs_ct_cap;
   // Engine numbering of each caller vertex id, or null when the engine keeps
   // the caller's numbering.  Update records carry caller ids.
   const int32_t* vmap;
@@ -346,10 +355,16 @@ struct Stage {
 // directly (wq_append), which reserves its own slot and chunks in one atomic
 // and keeps the same order property.
 constexpr int kWideStage = 256;
+struct WideStageMem {
+  int32_t vert[kWideStage];    // staged vertex
+  int32_t chunks[kWideStage];  // its chunk count
+  int32_t lo[kWideStage];      // its arcs [lo, hi)
+  int32_t hi[kWideStage];
+  int32_t base[kWideStage];    // its first chunk in the queue, set by wide_flush
+  int32_t cnt;
+};
 struct WideStage {
-  int32_t* vert;    // LDS, kWideStage
-  int32_t* chunks;  // LDS, kWideStage
-  int32_t* cnt;     // LDS
+  WideStageMem* m;  // LDS
 };
 
 __device__ __forceinline__ int wave_incl_scan_i32(int x) {
@@ -385,10 +400,12 @@ struct WList {
   const int32_t* base;
   int slots;
   int chunks;
+  int q;  // queue index, -1 for the static lists
 };
 
 __device__ __forceinline__ WList wlist(const Dev& D, int q, unsigned long long packed) {
   WList L;
+  L.q = q;
   L.vert = D.wq_vert + static_cast<size_t>(q) * D.wq_cap;
   L.base = D.wq_base + static_cast<size_t>(q) * D.wq_cap;
   L.slots = static_cast<int>(packed >> 32);
@@ -397,6 +414,7 @@ __device__ __forceinline__ WList wlist(const Dev& D, int q, unsigned long long p
 }
 __device__ __forceinline__ WList wlist_waves(const Dev& D) {
   WList L;
+  L.q = -1;
   L.vert = D.wave_vert;
   L.base = D.wave_base;
   L.slots = static_cast<int>(D.wave_packed >> 32);
@@ -405,6 +423,7 @@ __device__ __forceinline__ WList wlist_waves(const Dev& D) {
 }
 __device__ __forceinline__ WList wlist_static(const Dev& D) {
   WList L;
+  L.q = -1;
   L.vert = D.wide_vert;
   L.base = D.wide_base;
   L.slots = static_cast<int>(D.wide_packed >> 32);
@@ -418,6 +437,36 @@ __device__ __forceinline__ void wq_append(const Dev& D, int q, int v, int deg) {
   const size_t slot = static_cast<size_t>(old >> 32);
   D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
   D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = static_cast<int>(old & 0xffffffffu);
+}
+
+// Entries [base, base + chunks) of BFS ring queue q's chunk table for vertex
+// v with arcs [lo, hi).  Entries past the table (impossible while every
+// vertex is appended at most once per ring slot) fail the solve instead of
+// being written.  One lane.
+__device__ __forceinline__ void ct_write(const Dev& D, int q, int base, int v, int lo, int hi) {
+  if (!D.bfs_ct) return;
+  const int chunks = (hi - lo + kChunk - 1) / kChunk;
+  if (base + chunks > D.bfs_ct_cap) {
+    D.ctrl->status = DMF_EDEVICE;
+    return;
+  }
+  int4* ct = D.bfs_ct + static_cast<size_t>(q) * D.bfs_ct_cap;
+  for (int k = 0; k < chunks; ++k) {
+    const int a = lo + k * kChunk;
+    ct[base + k] = make_int4(v, a, min(a + kChunk, hi), 0);
+  }
+}
+
+// wq_append into BFS ring queue q, with the vertex's chunk-table entries.
+// One lane.
+__device__ __forceinline__ void bfs_wq_append(const Dev& D, int q, int v, int lo, int hi) {
+  const unsigned chunks = static_cast<unsigned>((hi - lo + kChunk - 1) / kChunk);
+  const unsigned long long old = atomicAdd(&D.ctrl->wq[q], (1ull << 32) | chunks);
+  const size_t slot = static_cast<size_t>(old >> 32);
+  const int base = static_cast<int>(old & 0xffffffffu);
+  D.wq_vert[static_cast<size_t>(q) * D.wq_cap + slot] = v;
+  D.wq_base[static_cast<size_t>(q) * D.wq_cap + slot] = base;
+  ct_write(D, q, base, v, lo, hi);
 }
 
 // Lanes with pred set append vertex v (degree deg) to wide queue q with one
@@ -446,53 +495,79 @@ __device__ __forceinline__ void wave_wq_append(const Dev& D, int q, bool pred, i
                    D.wq_base + static_cast<size_t>(q) * D.wq_cap, pred, v, deg);
 }
 
-// Lanes with pred set note wide vertex v (degree deg) into the block stage.
-// All 64 lanes.
+// Lanes with pred set note wide vertex v into the block stage (past
+// kWideStage entries they append it directly).  All 64 lanes.
 __device__ __forceinline__ void wide_note(const Dev& D, const WideStage& S, int q, bool pred, int v) {
   const unsigned long long b = __ballot(pred);
   if (b == 0) return;
   const int leader = __ffsll(static_cast<long long>(b)) - 1;
   int base = 0;
-  if (lane_id() == leader) base = atomicAdd(S.cnt, __popcll(b));
+  if (lane_id() == leader) base = atomicAdd(&S.m->cnt, __popcll(b));
   base = __shfl(base, leader);
   if (pred) {
-    const int deg = D.row[v + 1] - D.row[v];
+    const int lo = D.row[v], hi = D.row[v + 1];
     const int idx = base + __popcll(b & ((1ull << lane_id()) - 1ull));
     if (idx < kWideStage) {
-      S.vert[idx] = v;
-      S.chunks[idx] = (deg + kChunk - 1) / kChunk;
+      S.m->vert[idx] = v;
+      S.m->chunks[idx] = (hi - lo + kChunk - 1) / kChunk;
+      S.m->lo[idx] = lo;
+      S.m->hi[idx] = hi;
+    } else if (q < kNumRing) {
+      bfs_wq_append(D, q, v, lo, hi);
     } else {
-      wq_append(D, q, v, deg);
+      wq_append(D, q, v, hi - lo);
     }
   }
 }
-// Publishes the block's staged wide vertices into queue q.  Every thread of
-// the block must call it; the caller has made the stage visible
-// (__syncthreads) since the last wide_note.
+// Publishes the block's staged wide vertices into queue q: wave 0 reserves
+// their slots and chunks with one atomic and writes the slots; for a BFS ring
+// queue the whole block then writes the staged vertices' chunk-table entries,
+// each thread finding the vertex of its entries by a binary search over the
+// stage's chunk bases.  Every thread of the block must call it; the caller has
+// made the stage visible (__syncthreads) since the last wide_note.
 __device__ __forceinline__ void wide_flush(const Dev& D, const WideStage& S, int q) {
-  if (threadIdx.x >= kWave) return;
-  const int nv = min(*S.cnt, kWideStage);
-  if (nv == 0) return;
-  int sum = 0;
-  for (int i = lane_id(); i < nv; i += kWave) sum += S.chunks[i];
-  sum = __shfl(wave_incl_scan_i32(sum), kWave - 1);
-  unsigned long long old = 0;
-  if (lane_id() == 0)
-    old = atomicAdd(&D.ctrl->wq[q], (static_cast<unsigned long long>(nv) << 32) | static_cast<unsigned>(sum));
-  old = __shfl(old, 0);
-  int slot = static_cast<int>(old >> 32);
-  int run = static_cast<int>(old & 0xffffffffu);
-  int32_t* vert = D.wq_vert + static_cast<size_t>(q) * D.wq_cap;
-  int32_t* base = D.wq_base + static_cast<size_t>(q) * D.wq_cap;
-  for (int i0 = 0; i0 < nv; i0 += kWave) {
-    const int i = i0 + lane_id();
-    const int c = i < nv ? S.chunks[i] : 0;
-    const int incl = wave_incl_scan_i32(c);
-    if (i < nv) {
-      vert[slot + i] = S.vert[i];
-      base[slot + i] = run + incl - c;
+  const int nv = min(S.m->cnt, kWideStage);
+  if (nv == 0) return;  // block-uniform: read from LDS after the caller's barrier
+  if (threadIdx.x < kWave) {
+    int sum = 0;
+    for (int i = lane_id(); i < nv; i += kWave) sum += S.m->chunks[i];
+    sum = __shfl(wave_incl_scan_i32(sum), kWave - 1);
+    unsigned long long old = 0;
+    if (lane_id() == 0)
+      old = atomicAdd(&D.ctrl->wq[q], (static_cast<unsigned long long>(nv) << 32) | static_cast<unsigned>(sum));
+    old = __shfl(old, 0);
+    int slot = static_cast<int>(old >> 32);
+    int run = static_cast<int>(old & 0xffffffffu);
+    int32_t* vert = D.wq_vert + static_cast<size_t>(q) * D.wq_cap;
+    int32_t* base = D.wq_base + static_cast<size_t>(q) * D.wq_cap;
+    for (int i0 = 0; i0 < nv; i0 += kWave) {
+      const int i = i0 + lane_id();
+      const int c = i < nv ? S.m->chunks[i] : 0;
+      const int incl = wave_incl_scan_i32(c);
+      if (i < nv) {
+        vert[slot + i] = S.m->vert[i];
+        base[slot + i] = run + incl - c;
+        S.m->base[i] = run + incl - c;
+      }
+      run += __shfl(incl, kWave - 1);
     }
-    run += __shfl(incl, kWave - 1);
+  }
+  if (q >= kNumRing || !D.bfs_ct) return;  // uniform
+  __syncthreads();
+  const int first = S.m->base[0];
+  const int total = S.m->base[nv - 1] + S.m->chunks[nv - 1] - first;
+  if (first + total > D.bfs_ct_cap) {  // uniform; see ct_write
+    if (threadIdx.x == 0) D.ctrl->status = DMF_EDEVICE;
+    return;
+  }
+  int4* ct = D.bfs_ct + static_cast<size_t>(q) * D.bfs_ct_cap;
+  for (int e = threadIdx.x; e < total; e += kBlock) {
+    int i = 0;  // last staged vertex whose first chunk is at or before entry e
+#pragma unroll
+    for (int sp = kWideStage / 2; sp > 0; sp >>= 1)
+      if (i + sp < nv && S.m->base[i + sp] - first <= e) i += sp;
+    const int a = S.m->lo[i] + (e - (S.m->base[i] - first)) * kChunk;
+    ct[first + e] = make_int4(S.m->vert[i], a, min(a + kChunk, S.m->hi[i]), 0);
   }
 }
 
@@ -688,10 +763,10 @@ __device__ __forceinline__ int degree(const Dev& D, int v) { return D.row[v + 1]
 // block must call it.
 __device__ void ph_bfs_init(const Dev& D, const Range& r, int mode, Tally& t, bool early = false) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
-  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
+  __shared__ WideStageMem wsm;
   Stage st{sbuf[threadIdx.x / kWave], 0};
-  const WideStage WS{wvert, wchunks, &wcnt};
-  if (threadIdx.x == 0) wcnt = 0;
+  const WideStage WS{&wsm};
+  if (threadIdx.x == 0) wsm.cnt = 0;
   __syncthreads();
   const int n = D.n;
   long long targets = 0;
@@ -765,19 +840,14 @@ __device__ void ph_bfs_init_final(const Dev& D, const Range& r, int cnt, Tally& 
       D.lev16[v] = root ? 0 : kLevFree;
       t.vert += 1;
       if (root) {
-        const int d = degree(D, v);
-        if (d > D.wide_t) wq_append(D, 0, v, d);
+        const int a = D.row[v], b = D.row[v + 1];
+        if (b - a > D.wide_t) bfs_wq_append(D, 0, v, a, b);
       }
     }
     int rank;
     const int off = wave_reserve(&D.ctrl->qcnt[0], root, &rank);
     if (root) D.order[off + rank] = v;
   }
-}
-
-// A newly labelled vertex: wide ones also enter the next level's wide queue.
-__device__ __forceinline__ bool is_wide(const Dev& D, int v) {
-  return (D.widebits[static_cast<unsigned>(v) >> 5] >> (v & 31)) & 1u;
 }
 
 // A vertex is unlabelled while its height is n; a level claims it by
@@ -825,115 +895,224 @@ struct BfsTiles {
 // order[lo, lo+cnt) plus the chunks of wide queue W; newly labelled vertices
 // get height lvl+1 (one winner per vertex) and are appended after the frontier
 // through per-wave LDS staging, published per block at the end (wide ones
-// through the block's wide stage).  The 16 narrow frontier vertices a wave
-// takes at a time are expanded as one flattened arc range, so consecutive
-// lanes read consecutive arcs (coalesced col/residual loads) and each lane
-// finds its arc's tail by a 4-step binary search over the group's degree
-// prefix sums.  A step covers kBfsTiles tiles: the next step's heads and
-// residuals are loaded before the current step's labels are tested, and all
-// of a step's claims are issued before any of their results is waited for,
-// so each wave keeps several dependent round trips' worth of arcs in flight.
+// through the block's wide stage).
+//  * Wide frontier vertices: their chunks are spread over every wave of the
+//    grid.  A chunk's vertex and arc range come from the ring queue's chunk
+//    table with one load, issued a chunk ahead (without a table: a search
+//    over the queue's slots, then the vertex's row).
+//  * Narrow ones: the 16 a wave takes at a time are expanded as one flattened
+//    arc range, so consecutive lanes read consecutive arcs (coalesced
+//    col/residual loads) and each lane finds its arc's tail by a 4-step binary
+//    search over the group's degree prefix sums.
+// A step covers kBfsTiles tiles: the next step's heads and residuals are
+// loaded before the current step's labels are tested.  Claims are pipelined
+// one step deep: a step's compare-and-swaps (and the wide-vertex bits of their
+// heads, read beside them) are issued without waiting, and their results are
+// taken after the next step's label reads are out, so a claim's round trip
+// overlaps the next step's gathers instead of lengthening the wave's chain of
+// dependent loads.  A labelled head's lev16 is written one step later, which
+// only lets a few more claims fail.
 // early: count the targets this level labels (early-exit relabel).
 __device__ void ph_bfs_level(const Dev& D, const Range& r, int mode, int lo, int cnt, int lvl,
                              int32_t* next_cnt, const WList& W, int qnext, Tally& t, bool early = false) {
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
-  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
+  __shared__ WideStageMem wsm;
   Stage st{sbuf[threadIdx.x / kWave], 0};
-  const WideStage WS{wvert, wchunks, &wcnt};
-  if (threadIdx.x == 0) wcnt = 0;
+  const WideStage WS{&wsm};
+  if (threadIdx.x == 0) wsm.cnt = 0;
   __syncthreads();
   const int nb = lvl + 1;
   const uint16_t nb16 = static_cast<uint16_t>(nb < kLevFree ? nb : kLevFree - 1);
   int32_t* out = D.order + lo + cnt;
   long long found = 0;
+  // claims in flight: head, old height (-1: none), its widebits word, target flag
+  int pw[kBfsTiles], pold[kBfsTiles];
+  uint32_t pwb[kBfsTiles];
+  int ptg[kBfsTiles];
+#pragma unroll
+  for (int j = 0; j < kBfsTiles; ++j) {
+    pw[j] = 0;
+    pold[j] = -1;
+    pwb[j] = 0;
+    ptg[j] = 0;
+  }
+  // takes the results of the claims in flight (all 64 lanes)
+  auto settle = [&]() {
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) {
+      const bool hit = pold[j] == D.n;
+      if (hit) D.lev16[pw[j]] = nb16;
+      if (hit && early) found += ptg[j];
+      wide_note(D, WS, qnext, hit && ((pwb[j] >> (pw[j] & 31)) & 1u), pw[j]);
+      st.push(hit, pw[j], next_cnt, out);
+      pold[j] = -1;
+    }
+  };
   auto expand = [&](const BfsTiles& T) {
     bool c[kBfsTiles];
 #pragma unroll
     for (int j = 0; j < kBfsTiles; ++j) c[j] = T.in[j] && T.r[j] > 0 && bfs_head_ok(D, mode, T.w[j]);
 #pragma unroll
     for (int j = 0; j < kBfsTiles; ++j) c[j] = c[j] && bfs_unlabelled(D, T.w[j]);
-    int old[kBfsTiles];
+    settle();
 #pragma unroll
     for (int j = 0; j < kBfsTiles; ++j) {
+      const int w = T.w[j];
       t.arc += T.in[j] ? 1 : 0;
       t.atom += c[j] ? 1 : 0;
-      old[j] = c[j] ? atomicCAS(&D.height[T.w[j]], D.n, nb) : -1;
-    }
-#pragma unroll
-    for (int j = 0; j < kBfsTiles; ++j) {
-      const bool hit = old[j] == D.n;
-      if (hit) D.lev16[T.w[j]] = nb16;
-      if (hit && early) found += D.tgt[T.w[j]];
-      wide_note(D, WS, qnext, hit && is_wide(D, T.w[j]), T.w[j]);
-      st.push(hit, T.w[j], next_cnt, out);
+      pw[j] = w;
+      pold[j] = c[j] ? atomicCAS(&D.height[w], D.n, nb) : -1;
+      pwb[j] = c[j] ? D.widebits[static_cast<unsigned>(w) >> 5] : 0u;
+      if (early) ptg[j] = c[j] ? D.tgt[w] : 0;
     }
   };
   constexpr int kStep = kBfsTiles * kWave;
-  // wide frontier vertices: chunks spread over every wave of the grid
-  for (int c = r.wid; c < W.chunks; c += r.nw) {
-    int a, b;
-    const int v = chunk_range(D, W, c, &a, &b);
+  // heads and residuals of arcs [j0, min(j0 + kStep, b)) of wide vertex v
+  auto wfetch = [&](int v, int j0, int b) {
     const bool vB = mode == BFS_PP && D.side[v] != 0;
-    auto fetch = [&](int j0) {
-      BfsTiles T;
+    BfsTiles T;
 #pragma unroll
-      for (int j = 0; j < kBfsTiles; ++j) {
-        const int e = j0 + j * kWave + lane_id();
-        T.in[j] = e < b;
-        T.w[j] = T.in[j] ? D.col[e] : 0;
-        T.r[j] = T.in[j] ? bfs_resid(D, mode, vB, e) : 0;
+    for (int j = 0; j < kBfsTiles; ++j) {
+      const int e = j0 + j * kWave + lane_id();
+      T.in[j] = e < b;
+      T.w[j] = T.in[j] ? D.col[e] : 0;
+      T.r[j] = T.in[j] ? bfs_resid(D, mode, vB, e) : 0;
+    }
+    return T;
+  };
+  const int4* ct = (D.bfs_ct && W.q >= 0 && W.q < kNumRing)
+                       ? D.bfs_ct + static_cast<size_t>(W.q) * D.bfs_ct_cap : nullptr;
+  if (ct) {
+    // the wave's chunks c = wid, wid + nw, ... as one stream of steps: the
+    // next step's loads are issued before the current step is expanded also
+    // across chunk boundaries, and the table entry of the chunk after the
+    // next is loaded a chunk ahead
+    int c = r.wid;
+    if (c < W.chunks) {
+      int4 ch = ct[c];
+      int4 nx = c + r.nw < W.chunks ? ct[c + r.nw] : make_int4(0, 0, 0, 0);
+      int j0 = ch.y;
+      BfsTiles cur = wfetch(ch.x, j0, ch.z);
+      for (;;) {
+        int nj0 = j0 + kStep;
+        bool more = true;
+        if (nj0 >= ch.z) {
+          c += r.nw;
+          more = c < W.chunks;
+          if (more) {
+            ch = nx;
+            nj0 = ch.y;
+            if (c + r.nw < W.chunks) nx = ct[c + r.nw];
+          }
+        }
+        const BfsTiles now = cur;
+        if (more) cur = wfetch(ch.x, nj0, ch.z);
+        expand(now);
+        if (!more) break;
+        j0 = nj0;
       }
-      return T;
-    };
-    BfsTiles cur = fetch(a);
-    for (int j0 = a; j0 < b; j0 += kStep) {
-      const BfsTiles now = cur;
-      if (j0 + kStep < b) cur = fetch(j0 + kStep);
-      expand(now);
+    }
+  } else {
+    for (int c = r.wid; c < W.chunks; c += r.nw) {
+      int a, b;
+      const int v = chunk_range(D, W, c, &a, &b);
+      BfsTiles cur = wfetch(v, a, b);
+      for (int j0 = a; j0 < b; j0 += kStep) {
+        const BfsTiles now = cur;
+        if (j0 + kStep < b) cur = wfetch(v, j0 + kStep, b);
+        expand(now);
+      }
     }
   }
-  for (int base = r.wid * kGroup; base < cnt; base += r.nw * kGroup) {
-    const int i = base + lane_id();
-    int b = 0, deg = 0;
-    int vB = 0;
-    if (lane_id() < kGroup && i < cnt) {
-      const int v = D.order[lo + i];
-      b = D.row[v];
-      deg = D.row[v + 1] - b;
-      if (deg > D.wide_t) deg = 0;  // handled through its chunks
-      vB = (mode == BFS_PP && D.side[v] != 0) ? 1 : 0;
-      t.vert += 1;
-    }
-    const int incl = wave_incl_scan_i32(deg);
-    const int total = __shfl(incl, kWave - 1);
-    const int excl = incl - deg;
-    // called by the whole wave (shuffles) with a wave-uniform k0
-    auto fetch = [&](int k0) {
-      BfsTiles T;
-#pragma unroll
-      for (int j = 0; j < kBfsTiles; ++j) {
-        const int p = k0 + j * kWave + lane_id();
-        int pos = 0;  // owner lane: number of group lanes whose prefix end <= p
-#pragma unroll
-        for (int sh = kGroup / 2; sh > 0; sh >>= 1)
-          if (__shfl(incl, pos + sh - 1) <= p) pos += sh;
-        pos &= kGroup - 1;
-        const int e = __shfl(b, pos) + (p - __shfl(excl, pos));
-        const bool ok = __shfl(vB, pos) != 0;
-        T.in[j] = p < total;
-        T.w[j] = T.in[j] ? D.col[e] : 0;
-        T.r[j] = T.in[j] ? bfs_resid(D, mode, ok, e) : 0;
+  // Narrow groups: lane i < kGroup holds the group's i-th vertex.  The
+  // groups c = wid, wid + nw, ... form one stream of steps like the wide
+  // chunks: the next group's first loads are issued before the current
+  // group's last step is expanded, the adjacency bounds of the group after it
+  // are loaded one group ahead, and its order[] entries two groups ahead.
+  struct Grp {
+    int b, incl, excl, total, vB;
+  };
+  const int gstride = r.nw * kGroup;
+  int gbase = r.wid * kGroup;  // base of the group whose bounds are in (cv, cb, ce)
+  auto order_at = [&](int g) {
+    const int i = g + lane_id();
+    return lane_id() < kGroup && i < cnt ? D.order[lo + i] : -1;
+  };
+  int cv = order_at(gbase);
+  int cb = cv >= 0 ? D.row[cv] : 0, ce = cv >= 0 ? D.row[cv + 1] : 0;
+  int nv = order_at(gbase + gstride);
+  // the next group holding narrow arcs (total 0: none left); all 64 lanes
+  auto next_group = [&]() {
+    Grp G;
+    for (;;) {
+      if (gbase >= cnt) {
+        G.b = G.incl = G.excl = G.total = G.vB = 0;
+        return G;
       }
-      return T;
-    };
-    if (total == 0) continue;
-    BfsTiles cur = fetch(0);
-    for (int k0 = 0; k0 < total; k0 += kStep) {
-      const BfsTiles now = cur;
-      if (k0 + kStep < total) cur = fetch(k0 + kStep);
-      expand(now);
+      const int v = cv, b = cb, e = ce;
+      cb = nv >= 0 ? D.row[nv] : 0;
+      ce = nv >= 0 ? D.row[nv + 1] : 0;
+      cv = nv;
+      nv = order_at(gbase + 2 * gstride);
+      gbase += gstride;
+      int deg = 0;
+      G.b = 0;
+      G.vB = 0;
+      if (v >= 0) {
+        deg = e - b;
+        if (deg > D.wide_t) deg = 0;  // handled through its chunks
+        G.b = b;
+        G.vB = (mode == BFS_PP && D.side[v] != 0) ? 1 : 0;
+        t.vert += 1;
+      }
+      G.incl = wave_incl_scan_i32(deg);
+      G.total = __shfl(G.incl, kWave - 1);
+      G.excl = G.incl - deg;
+      if (G.total > 0) return G;
+    }
+  };
+  // heads and residuals of flattened arcs [k0, k0 + kStep) of group G; all
+  // 64 lanes (shuffles), k0 wave-uniform
+  auto gfetch = [&](const Grp& G, int k0) {
+    BfsTiles T;
+#pragma unroll
+    for (int j = 0; j < kBfsTiles; ++j) {
+      const int p = k0 + j * kWave + lane_id();
+      int pos = 0;  // owner lane: number of group lanes whose prefix end <= p
+#pragma unroll
+      for (int sh = kGroup / 2; sh > 0; sh >>= 1)
+        if (__shfl(G.incl, pos + sh - 1) <= p) pos += sh;
+      pos &= kGroup - 1;
+      const int e = __shfl(G.b, pos) + (p - __shfl(G.excl, pos));
+      const bool ok = __shfl(G.vB, pos) != 0;
+      T.in[j] = p < G.total;
+      T.w[j] = T.in[j] ? D.col[e] : 0;
+      T.r[j] = T.in[j] ? bfs_resid(D, mode, ok, e) : 0;
+    }
+    return T;
+  };
+  {
+    Grp G = next_group();
+    if (G.total > 0) {
+      int k0 = 0;
+      BfsTiles cur = gfetch(G, 0);
+      for (;;) {
+        int nk0 = k0 + kStep;
+        bool more = true;
+        if (nk0 >= G.total) {
+          G = next_group();
+          nk0 = 0;
+          more = G.total > 0;
+        }
+        const BfsTiles now = cur;
+        if (more) cur = gfetch(G, nk0);
+        expand(now);
+        if (!more) break;
+        k0 = nk0;
+      }
     }
   }
+  settle();
   if (early) {
     found = wave_sum_i64(found);
     if (lane_id() == 0 && found) atomicAdd(&D.ctrs[C_TGTF * kSlots + (r.wid % kSlots)], found);
@@ -1019,7 +1198,7 @@ __device__ void ph_bfs_bottomup(const Dev& D, const Range& r, int lvl, int32_t* 
       // w is owned by this thread alone: a plain store labels it
       D.height[w] = lvl + 1;
       D.lev16[w] = static_cast<uint16_t>(lvl + 1 < kLevFree ? lvl + 1 : kLevFree - 1);
-      if (deg > D.wide_t) wq_append(D, qnext, w, deg);
+      if (deg > D.wide_t) bfs_wq_append(D, qnext, w, b, b + deg);
       if (early) found += D.tgt[w];
     }
     st.push(hit, w, next_cnt, out);
@@ -1078,10 +1257,10 @@ __device__ void ph_build_wl(const Dev& D, const Range& r, int cnt, Tally& t) {
   // active with a valid label).  Appends are staged in LDS and published per
   // block; every thread of the block must call it.
   __shared__ int32_t sbuf[kBlock / kWave][kStageCap];
-  __shared__ int32_t wvert[kWideStage], wchunks[kWideStage], wcnt;
+  __shared__ WideStageMem wsm;
   Stage st{sbuf[threadIdx.x / kWave], 0};
-  const WideStage WS{wvert, wchunks, &wcnt};
-  if (threadIdx.x == 0) wcnt = 0;
+  const WideStage WS{&wsm};
+  if (threadIdx.x == 0) wsm.cnt = 0;
   __syncthreads();
   const int total = cnt < 0 ? D.n : cnt;
   for (int base = r.gid - lane_id(); base < total; base += r.gsz) {

@@ -135,6 +135,10 @@ struct dmf_engine {
   // per-launch BFS level timing (dmf_engine_set_profiling)
   bool profile = false;
   hipEvent_t pev0 = nullptr, pev1 = nullptr;
+  // push-pull rounds: the A side's pull and repair run on a second stream
+  // beside the B side's push and repair (created on first use)
+  hipStream_t pull_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -179,13 +183,14 @@ int sync_ctrl(dmf_engine* E) {
   return DMF_OK;
 }
 
-#define LAUNCH(kernel, blocks, ...)                                                   \
+#define LAUNCH_ON(st, kernel, blocks, ...)                                            \
   do {                                                                                \
-    hipLaunchKernelGGL(kernel, dim3(std::min<int>((blocks), E->grid_cap)), dim3(kBlock), 0, E->stream, \
+    hipLaunchKernelGGL(kernel, dim3(std::min<int>((blocks), E->grid_cap)), dim3(kBlock), 0, (st), \
                        __VA_ARGS__);                                                  \
     ++E->launches;                                                                    \
     HIP_OK(hipGetLastError());                                                        \
   } while (0)
+#define LAUNCH(kernel, blocks, ...) LAUNCH_ON(E->stream, kernel, blocks, __VA_ARGS__)
 
 int zero_dev(dmf_engine* E, void* p, size_t bytes) {
   HIP_OK(hipMemsetAsync(p, 0, bytes, E->stream));
@@ -431,6 +436,16 @@ int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int 
   return DMF_OK;
 }
 
+// The second stream of push-pull rounds and its fork/join events, created on
+// first use (default priority, like the engine's own stream).
+int ensure_pull_stream(dmf_engine* E) {
+  if (E->pull_stream) return DMF_OK;
+  HIP_OK(hipStreamCreateWithFlags(&E->pull_stream, hipStreamNonBlocking));
+  HIP_OK(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
+  return DMF_OK;
+}
+
 // Stepped solve (one kernel per phase, host-checked conditions).
 int solve_stepped(dmf_engine* E, const Cfg& C0) {
   Dev& D = E->D;
@@ -633,11 +648,31 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         conv = false;
         break;
       }
-      if (np || wp) LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
-      if (nq || wq) LAUNCH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+      const bool side_b = np || wp, side_a = nq || wq;
+      if (side_a && side_b) {
+        if ((rc = ensure_pull_stream(E))) return rc;
+        // The reference pulls on one stream and pushes on another
+        // (code/dyn_pp_bench.cu:42-60).  The B side's push and repair touch
+        // only arcs and excesses between B vertices, the A side's pull and
+        // repair only those between A vertices (the cut between them is
+        // saturated and side[] does not change within a round), so the two
+        // chains run at once: A on the second stream, forked from and joined
+        // back into the engine's stream.
+        HIP_OK(hipEventRecord(E->ev_fork, E->stream));
+        HIP_OK(hipStreamWaitEvent(E->pull_stream, E->ev_fork, 0));
+        LAUNCH_ON(E->pull_stream, k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+        LAUNCH_ON(E->pull_stream, k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
+        HIP_OK(hipEventRecord(E->ev_join, E->pull_stream));
+        LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
+        LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
+        HIP_OK(hipStreamWaitEvent(E->stream, E->ev_join, 0));
+      } else {
+        if (side_b) LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
+        if (side_a) LAUNCH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+        if (side_b) LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
+        if (side_a) LAUNCH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
+      }
       active = static_cast<double>(np) + nq + static_cast<double>(wp >> 32) + static_cast<double>(wq >> 32);
-      if (np || wp) LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
-      if (nq || wq) LAUNCH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
       ++round;
       ++since;
       if (consult) {
@@ -1534,6 +1569,9 @@ int dmf_engine_destroy(dmf_engine* E) {
     if (E->pev0) (void)hipEventDestroy(E->pev0);
     if (E->pev1) (void)hipEventDestroy(E->pev1);
     if (E->ev1) hipEventDestroy(E->ev1);
+    if (E->ev_fork) (void)hipEventDestroy(E->ev_fork);
+    if (E->ev_join) (void)hipEventDestroy(E->ev_join);
+    if (E->pull_stream) (void)hipStreamDestroy(E->pull_stream);
     if (E->own_stream && E->stream) hipStreamDestroy(E->stream);
     delete E;
     return DMF_OK;

@@ -40,8 +40,9 @@ BYTES_ARC, BYTES_VERTEX, BYTES_UPDATE = 12, 16, 24
 # WRITE_SIZE per dispatch, scaled by a plain copy of known size); written from
 # the committed profile summary of this workload
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bfs_level_dev_rmat20_traffic.json")
-DRIVERS = {0: "persistent", 1: "stepped", 2: "stepped, BFS levels replayed from hipGraphs"}
-LEVEL_KERNEL = {0: "k_solve_persistent", 1: "k_bfs_level", 2: "k_bfs_level_dev"}
+DRIVERS = {0: "persistent", 1: "stepped", 2: "stepped, BFS levels replayed from hipGraphs",
+           3: "round graphs (each fixed-schedule round replayed from one hipGraph)"}
+LEVEL_KERNEL = {0: "k_solve_persistent", 1: "k_bfs_level", 2: "k_bfs_level_dev", 3: "k_bfs_level_dev"}
 
 
 def log(*a):
@@ -375,7 +376,8 @@ def main():
     ap.add_argument("--ef", type=int, default=16)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--mode", type=int, default=2,
-                    help="driver: 0 persistent one-launch solve, 1 stepped, 2 stepped with graph-replayed BFS levels")
+                    help="driver: 0 persistent one-launch solve, 1 stepped, 2 stepped with graph-replayed BFS levels, "
+                         "3 round graphs (fixed schedule only; others run as 2)")
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels of the plain passes")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")

@@ -132,6 +132,15 @@ struct dmf_engine {
   bool use_graph = false;
   bool graph_failed = false;
   int seg_levels[kBfsModes] = {};
+  // round graphs (mode 3): a whole data-driven round with 2*i level launches
+  // in slot i, captured for push/relabel cycles kc (a kernel argument fixed
+  // at capture); round_depth is the deepest relabel of the last such solve
+  static constexpr int kRoundSlots = 33;  // up to 64 level launches
+  hipGraph_t round_graph[kRoundSlots] = {};
+  hipGraphExec_t round_exec[kRoundSlots] = {};
+  int round_kc[kRoundSlots] = {};
+  int round_depth = 0;
+  bool use_rounds = false;
   // per-launch BFS level timing (dmf_engine_set_profiling)
   bool profile = false;
   hipEvent_t pev0 = nullptr, pev1 = nullptr;
@@ -295,6 +304,66 @@ void destroy_segments(dmf_engine* E) {
       E->seg_exec[a][b] = nullptr;
       E->seg_graph[a][b] = nullptr;
     }
+  for (int b = 0; b < dmf_engine::kRoundSlots; ++b) {
+    if (E->round_exec[b]) (void)hipGraphExecDestroy(E->round_exec[b]);
+    if (E->round_graph[b]) (void)hipGraphDestroy(E->round_graph[b]);
+    E->round_exec[b] = nullptr;
+    E->round_graph[b] = nullptr;
+  }
+}
+
+// Kernels in one captured round of slot idx: begin, roots, 2*idx levels, mid,
+// worklist, list, push, repair.
+int round_launches(int idx) { return 7 + 2 * idx; }
+// Slot whose round holds at least depth + 1 level launches (a relabel of
+// depth d takes d + 1: the last finds its frontier empty), capped at 64.
+int round_slot(int depth) { return std::min(dmf_engine::kRoundSlots - 1, std::max(1, (depth + 2) / 2)); }
+
+// Captures one data-driven round of the fixed schedule with 2*idx BFS level
+// launches (an even count, so the levels end on cursor slot 0, which
+// k_round_mid reads).  Grids are fixed at capture: every kernel strides over
+// whatever its list holds at run time.  The captured kernels take the engine
+// state by value, so the graphs are rebuilt whenever that state is
+// (setup_wide).
+int capture_round(dmf_engine* E, int idx, int kc) {
+  if (E->round_exec[idx] && E->round_kc[idx] == kc) return DMF_OK;
+  if (E->round_exec[idx]) {
+    (void)hipGraphExecDestroy(E->round_exec[idx]);
+    (void)hipGraphDestroy(E->round_graph[idx]);
+    E->round_exec[idx] = nullptr;
+    E->round_graph[idx] = nullptr;
+  }
+  const Dev& D = E->D;
+  const int gn = std::min(blocks_for(E->n), E->grid_cap);
+  const int glev = std::min(phase_blocks(E->n, chunks_of(D.wide_packed)), E->grid_cap);
+  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->grid_cap);
+  hipStream_t st = E->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
+  hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(kWave), 0, st, D);
+  hipLaunchKernelGGL(k_round_bfs_init, dim3(gn), dim3(kBlock), 0, st, D);
+  for (int i = 0; i < 2 * idx; ++i)
+    hipLaunchKernelGGL(k_bfs_level_dev, dim3(glev), dim3(kBlock), 0, st, E->d_dev, static_cast<int>(BFS_SINK), i & 1);
+  hipLaunchKernelGGL(k_round_mid, dim3(1), dim3(kWave), 0, st, D);
+  hipLaunchKernelGGL(k_round_wl, dim3(gn), dim3(kBlock), 0, st, D);
+  hipLaunchKernelGGL(k_round_list, dim3(1), dim3(kWave), 0, st, D);
+  hipLaunchKernelGGL(k_round_push, dim3(glist), dim3(kBlock), 0, st, D, kc);
+  hipLaunchKernelGGL(k_round_repair, dim3(glist), dim3(kBlock), 0, st, D);
+  const bool ok = hipGetLastError() == hipSuccess;
+  hipGraph_t g = nullptr;
+  const bool ended = hipStreamEndCapture(st, &g) == hipSuccess;
+  if (!ok || !ended || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return kNoGraph;
+  }
+  hipGraphExec_t x = nullptr;
+  if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return kNoGraph;
+  }
+  E->round_graph[idx] = g;
+  E->round_exec[idx] = x;
+  E->round_kc[idx] = kc;
+  return DMF_OK;
 }
 
 // Global relabel with the level loop replayed from captured graphs: one host
@@ -436,6 +505,79 @@ int run_sched(dmf_engine* E, int phase, int rounds_since, const char* what, int 
   return DMF_OK;
 }
 
+// The round-graph driver's data-driven loop (fixed schedule, no cascade
+// sub-rounds, early-exit relabels or lazy start): one replay per round and
+// one host check after it.  A relabel deeper than the captured level launches
+// stops the replay (r_stop); the host then runs further level segments from
+// the device cursor, finishes the round with the same kernels launched one by
+// one, and captures a longer round for the rounds after it.  The order buffer
+// is pinned to the first of the two (this driver never restricts bottom-up
+// levels to a previous labelled set).  Same phases, lists and conditions as
+// the host-checked loop of solve_stepped, so the same rounds and results.
+int rounds_loop(dmf_engine* E, const Cfg& C, int* round, int* visited, bool* conv) {
+  Dev& D = E->D;
+  int rc;
+  if (E->ord[0]) D.order = E->ord[0];
+  D.prev = nullptr;
+  D.nprev = 0;
+  E->full_buf = -1;
+  hipLaunchKernelGGL(k_round_init, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev, C.interval, C.max_rounds);
+  ++E->launches;
+  HIP_OK(hipGetLastError());
+  // rounds as long as the deepest relabel of the last solve of this driver
+  int idx = round_slot(E->round_depth);
+  const int gn = std::min(blocks_for(E->n), E->grid_cap);
+  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->grid_cap);
+  int deepest = 0;
+  for (;;) {
+    if ((rc = capture_round(E, idx, C.kc))) return rc;
+    HIP_OK(hipGraphLaunch(E->round_exec[idx], E->stream));
+    E->launches += static_cast<uint64_t>(round_launches(idx));
+    if ((rc = sync_ctrl(E))) return rc;
+    if (E->h_ctrl->r_stop) {
+      // the relabel goes on past the captured levels: segments of level
+      // launches from the cursor until the frontier is empty
+      int seg = 1;
+      while (seg + 1 < dmf_engine::kSegSizes && (1 << seg) < 2 * idx) ++seg;
+      for (;;) {
+        if ((rc = capture_segment(E, BFS_SINK, seg))) return rc;
+        HIP_OK(hipGraphLaunch(E->seg_exec[BFS_SINK][seg], E->stream));
+        E->launches += static_cast<uint64_t>(1) << seg;
+        if ((rc = sync_ctrl(E))) return rc;
+        const int lvl = E->h_ctrl->bcur[0][1];
+        if (E->h_ctrl->qcnt[lvl % 3] == 0) {
+          deepest = std::max(deepest, lvl);
+          break;
+        }
+        if (lvl > E->n + 1) return DMF_EDEVICE;
+        if (seg + 1 < dmf_engine::kSegSizes) ++seg;
+      }
+      hipLaunchKernelGGL(k_round_resume, dim3(1), dim3(kWave), 0, E->stream, D);
+      hipLaunchKernelGGL(k_round_mid, dim3(1), dim3(kWave), 0, E->stream, D);
+      hipLaunchKernelGGL(k_round_wl, dim3(gn), dim3(kBlock), 0, E->stream, D);
+      hipLaunchKernelGGL(k_round_list, dim3(1), dim3(kWave), 0, E->stream, D);
+      hipLaunchKernelGGL(k_round_push, dim3(glist), dim3(kBlock), 0, E->stream, D, C.kc);
+      hipLaunchKernelGGL(k_round_repair, dim3(glist), dim3(kBlock), 0, E->stream, D);
+      E->launches += 6;
+      HIP_OK(hipGetLastError());
+      idx = std::max(idx, round_slot(deepest));
+      continue;
+    }
+    if (E->h_ctrl->r_done) break;
+  }
+  const Ctrl& c = *E->h_ctrl;
+  *round = c.r_rounds;
+  *visited = c.visited;
+  *conv = c.r_noconv == 0;
+  E->last.bfs_levels += static_cast<uint64_t>(c.r_levels);
+  E->last.global_relabels += static_cast<uint64_t>(c.r_relabels);
+  // depth of the last relabel: the converging one
+  E->last_levels = c.bcur[0][1];
+  E->seg_levels[BFS_SINK] = std::max(c.bcur[0][1], 1);
+  E->round_depth = std::max(c.r_maxlvl, deepest);
+  return DMF_OK;
+}
+
 // The second stream of push-pull rounds and its fork/join events, created on
 // first use (default priority, like the engine's own stream).
 int ensure_pull_stream(dmf_engine* E) {
@@ -519,7 +661,23 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     if (!E->D.cmark) sub = 0;
     const bool try_early = !topo && E->D.tgt && (E->early_mode >= 2 || (E->early_mode == 1 && sub > 0));
     bool need_full = true;
-    for (;;) {
+    // round graphs: the fixed schedule's data-driven loop with its conditions
+    // on the device (other schedules keep the host-checked loop below)
+    bool rounds = E->use_rounds && !E->graph_failed && !topo && !consult && sub == 0 && !try_early && want_relabel;
+    if (rounds) {
+      rc = rounds_loop(E, C, &round, &visited, &conv);
+      if (rc == kNoGraph) {
+        // capture refused: the host-checked loop carries on from the current
+        // state (it starts with a relabel) on cleared lists
+        E->graph_failed = true;
+        std::fprintf(stderr, "dmf: round graph capture unavailable, using per-phase launches\n");
+        if ((rc = reset_lists(E))) return rc;
+        rounds = false;
+      } else if (rc) {
+        return rc;
+      }
+    }
+    for (; !rounds;) {
       bool fresh = false;
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
@@ -812,7 +970,7 @@ int finish_solve(dmf_engine* E, int rc_driver, int64_t* flow) {
 int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   Cfg C = make_cfg(E, p, entry);
   if (C.variant < 0 || C.variant > 2 || C.scheduler < 0 || C.scheduler > 1) return DMF_EINVAL;
-  if (p && (p->mode < 0 || p->mode > 2)) return DMF_EINVAL;
+  if (p && (p->mode < 0 || p->mode > 3)) return DMF_EINVAL;
   if (entry == 1 && C.variant == DMF_VARIANT_PP && !E->pp_ready) C.variant = DMF_VARIANT_DATA;
   int rc = begin_solve(E);
   if (rc) return rc;
@@ -820,9 +978,10 @@ int run_solve(dmf_engine* E, const dmf_params* p, int entry, int64_t* flow) {
   ++E->solve_id;
   E->t_solve = std::chrono::steady_clock::now();
   // the scheduler runs between rounds on the host-driven path
-  const bool stepped = (p && (p->mode == 1 || p->mode == 2)) || E->coop_blocks <= 0 ||
+  const bool stepped = (p && p->mode >= 1) || E->coop_blocks <= 0 ||
                        C.scheduler == DMF_SCHED_LEARNED || E->trace != nullptr;
-  E->use_graph = p && p->mode == 2;
+  E->use_graph = p && (p->mode == 2 || p->mode == 3);
+  E->use_rounds = p && p->mode == 3;
   if (!stepped) E->labels_fresh = false;
   int rc_driver = stepped ? solve_stepped(E, C) : solve_persistent(E, C);
   if (rc_driver == DMF_EDEVICE) return rc_driver;

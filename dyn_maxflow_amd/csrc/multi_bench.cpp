@@ -2,7 +2,7 @@
 //
 //   dmf_multi_bench [--instances=256] [--log-lo=10] [--log-hi=16] [--ef-lo=8]
 //                   [--ef-hi=32] [--seed=4] [--batches=5] [--pct=5]
-//                   [--repeats=3] [--mode=persistent|stepped|graph|all, or a comma list]
+//                   [--repeats=3] [--mode=persistent|stepped|graph|rounds|all, or a comma list]
 //                   [--variant=data|topo|pp] [--sched=plain|learned]
 //                   [--cascade=K] [--no-autotune] [--no-solo] [--json]
 //
@@ -275,7 +275,7 @@ int main(int argc, char** argv) {
   std::printf("batched engine: %lld device bytes\n", static_cast<long long>(dmf_engine_device_bytes(E)));
   std::vector<std::string> modes;
   if (o.mode == "all") {
-    modes = {"persistent", "stepped", "graph"};
+    modes = {"persistent", "stepped", "graph", "rounds"};
   } else {
     for (size_t a = 0; a <= o.mode.size();) {  // comma-separated list
       const size_t b = std::min(o.mode.find(',', a), o.mode.size());
@@ -284,7 +284,7 @@ int main(int argc, char** argv) {
     }
   }
   for (const std::string& mname : modes)
-    if (mname != "persistent" && mname != "stepped" && mname != "graph") {
+    if (mname != "persistent" && mname != "stepped" && mname != "graph" && mname != "rounds") {
       std::fprintf(stderr, "dmf_multi_bench: unknown mode %s\n", mname.c_str());
       return 2;
     }
@@ -295,7 +295,7 @@ int main(int argc, char** argv) {
     ModeResult R;
     R.name = mname;
     dmf_params q = p;
-    q.mode = mname == "persistent" ? 0 : mname == "stepped" ? 1 : 2;
+    q.mode = mname == "persistent" ? 0 : mname == "stepped" ? 1 : mname == "graph" ? 2 : 3;
     q.scheduler = o.sched;
     q.cascade = o.cascade;
     if (o.autotune) {

@@ -84,7 +84,24 @@ struct Ctrl {
   int32_t bfs_rbu;    // the current relabel ran a restricted bottom-up level
   int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
                       // read from slot r and written to slot r^1 by each level
+  // Round-graph driver: loop state of the data-driven fixed schedule kept on
+  // the device (see k_round_*).
+  int32_t r_done;      // the solve is over: converged, or out of rounds (r_noconv)
+  int32_t r_noconv;
+  int32_t r_stop;      // this round's relabel outgrew the captured level launches
+  int32_t r_relabel;   // this round starts with a global relabel
+  int32_t r_fresh;     // the labels come from this round's relabel
+  int32_t r_skip;      // the list came out empty on stale labels: no push this round
+  int32_t r_since;     // rounds since the last global relabel
+  int32_t r_interval;  // rounds between global relabels
+  int32_t r_rounds;    // push rounds so far
+  int32_t r_max;       // round cap
+  int32_t r_relabels;  // global relabels so far
+  int32_t r_levels;    // their BFS levels
+  int32_t r_maxlvl;    // the deepest of them
+  int32_t r_cnt;       // this round's narrow list size
   unsigned long long wq[kNumQ];  // wide queues: (slots << 32) | chunks
+  unsigned long long r_wpk;      // round-graph driver: this round's wide queue word
   long long flow;
 };
 
@@ -2317,6 +2334,148 @@ __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, in
   if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_waves(D), t);
   else if (which == 1) ph_repair(D, r, D.wl, cnt, rmode, wlist(D, kQActive, wpk), t);
   else ph_repair(D, r, D.wl2, cnt, rmode, wlist(D, kQPull, wpk), t);
+  t.flush(D.ctrs);
+}
+// ============================================================ round graphs
+// The round-graph driver replays a whole round of the data-driven fixed
+// schedule from one captured hipGraph: k_round_begin, k_round_bfs_init, a
+// fixed number of k_bfs_level_dev launches, k_round_mid, k_round_wl,
+// k_round_list, k_round_push, k_round_repair.  The loop conditions the
+// stepped driver checks on the host live in the control block (r_*): every
+// kernel reads them at its start and returns at once when the solve is over
+// (r_done) or when the host has to finish the round (r_stop), so the round's
+// launches are the stepped driver's, with one host check per replay instead
+// of one per BFS segment and one per worklist.
+
+// Once per solve, before the first replay: loop state, and the device copy of
+// Dev that k_bfs_level_dev reads.  One wave.
+__global__ void k_round_init(Dev D, Dev* out, int interval, int max_rounds) {
+  if (threadIdx.x != 0) return;
+  Ctrl* c = D.ctrl;
+  c->r_done = c->r_noconv = c->r_stop = c->r_relabel = c->r_fresh = c->r_skip = 0;
+  c->r_since = interval;  // the first round relabels
+  c->r_interval = interval;
+  c->r_rounds = 0;
+  c->r_max = max_rounds;
+  c->r_relabels = c->r_levels = c->r_maxlvl = 0;
+  c->r_cnt = 0;
+  c->r_wpk = 0;
+  c->early = c->bfs_cut = c->bfs_rbu = 0;
+  *out = D;
+}
+// Round start: whether this round relabels (every r_interval rounds, or at
+// once after a list that came out empty on stale labels); a relabel clears
+// the frontier ring and the level cursor.  One wave.
+__global__ void k_round_begin(Dev D) {
+  Ctrl* c = D.ctrl;
+  if (threadIdx.x != 0 || c->r_done || c->r_stop) return;
+  const int rl = c->r_since >= c->r_interval ? 1 : 0;
+  c->r_relabel = rl;
+  if (rl) {
+    c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+    c->wq[0] = c->wq[1] = c->wq[2] = 0;
+    c->bcur[0][0] = c->bcur[0][1] = c->bcur[1][0] = c->bcur[1][1] = 0;
+  }
+}
+// Roots of the round's relabel from t (my_init_1, code/dyn_data.h:60-67).
+__global__ void __launch_bounds__(kBlock) k_round_bfs_init(Dev D) {
+  const Ctrl* c = D.ctrl;
+  if (c->r_done || c->r_stop || !c->r_relabel) return;  // block-uniform
+  Tally t;
+  ph_bfs_init(D, make_range(), BFS_SINK, t);
+  t.flush(D.ctrs);
+}
+// After the level launches: a relabel whose frontier is not yet empty stops
+// the replay (r_stop; the host runs further levels, then k_round_resume and
+// the round's remaining kernels); a finished one gives the labelled set
+// order[0, visited).  Clears the worklist counters and the BFS ring for the
+// worklist builder.  One wave.
+__global__ void k_round_mid(Dev D) {
+  Ctrl* c = D.ctrl;
+  if (threadIdx.x != 0 || c->r_done || c->r_stop) return;
+  if (c->r_relabel) {
+    const int lvl = c->bcur[0][1];
+    if (c->qcnt[lvl % 3] != 0) {
+      c->r_stop = 1;
+      return;
+    }
+    c->visited = c->bcur[0][0];
+    c->r_levels += lvl;
+    c->r_maxlvl = max(c->r_maxlvl, lvl);
+    c->r_relabels += 1;
+    c->r_since = 0;
+    c->r_fresh = 1;
+  } else {
+    c->r_fresh = 0;
+  }
+  c->wl_cnt = 0;
+  c->wq[kQActive] = 0;
+  c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+  c->wq[0] = c->wq[1] = c->wq[2] = 0;
+}
+// Clears r_stop once the host has finished the relabel's levels.  One wave.
+__global__ void k_round_resume(Dev D) {
+  if (threadIdx.x == 0) D.ctrl->r_stop = 0;
+}
+// create_wl (code/dyn_data.h:217-238) over the labelled set of the last
+// relabel, or over every vertex when that set is a quarter of the graph or
+// more (one coalesced pass instead of a gather through order[]).
+__global__ void __launch_bounds__(kBlock) k_round_wl(Dev D) {
+  const Ctrl* c = D.ctrl;
+  if (c->r_done || c->r_stop) return;  // block-uniform
+  const int cnt = c->visited;
+  Tally t;
+  ph_build_wl(D, make_range(), cnt > D.n / 4 ? -1 : cnt, t);
+  t.flush(D.ctrs);
+}
+// The list decides the round: empty after this round's relabel ends the
+// solve (converged); empty on stale labels makes the next round relabel
+// without pushing in this one; past the round cap the solve ends without
+// convergence.  Otherwise the round's list is kept for push and repair.
+// One wave.
+__global__ void k_round_list(Dev D) {
+  Ctrl* c = D.ctrl;
+  if (threadIdx.x != 0 || c->r_done || c->r_stop) return;
+  const int cnt = c->wl_cnt;
+  const unsigned long long wpk = c->wq[kQActive];
+  c->r_cnt = cnt;
+  c->r_wpk = wpk;
+  c->r_skip = 0;
+  if (cnt == 0 && wpk == 0) {
+    if (c->r_fresh) {
+      c->r_done = 1;
+    } else {
+      c->r_since = c->r_interval;
+      c->r_skip = 1;
+    }
+    return;
+  }
+  if (c->r_rounds >= c->r_max) {
+    c->r_done = 1;
+    c->r_noconv = 1;
+    return;
+  }
+  c->r_rounds += 1;
+  c->r_since += 1;
+}
+// staticMaxFlow_kernel_14 (code/dyn_data.h:89-131) over the round's list.
+__global__ void __launch_bounds__(kBlock) k_round_push(Dev D, int kc) {
+  const Ctrl* c = D.ctrl;
+  if (c->r_done || c->r_stop || c->r_skip) return;  // block-uniform
+  const int cnt = c->r_cnt;
+  const unsigned long long wpk = c->r_wpk;
+  Tally t;
+  ph_push<false>(D, make_range(), D.wl, cnt, kc, PM_ALL, nullptr, wlist(D, kQActive, wpk), t);
+  t.flush(D.ctrs);
+}
+// staticMaxFlow_kernel_17 (code/dyn_data.h:132-153) over the round's list.
+__global__ void __launch_bounds__(kBlock) k_round_repair(Dev D) {
+  const Ctrl* c = D.ctrl;
+  if (c->r_done || c->r_stop || c->r_skip) return;  // block-uniform
+  const int cnt = c->r_cnt;
+  const unsigned long long wpk = c->r_wpk;
+  Tally t;
+  ph_repair(D, make_range(), D.wl, cnt, RM_ALL, wlist(D, kQActive, wpk), t);
   t.flush(D.ctrs);
 }
 __global__ void __launch_bounds__(kBlock) k_pp_cut(Dev D) {

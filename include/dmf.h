@@ -159,7 +159,14 @@ typedef struct dmf_params {
   int32_t max_rounds;       /* safety cap on rounds per solve (0 = default)     */
   int32_t mode;             /* 0 = persistent one-launch solve, 1 = stepped,
                                2 = stepped with the BFS levels replayed from
-                               captured hipGraph segments                     */
+                               captured hipGraph segments, 3 = round graphs:
+                               each data-driven round of the fixed schedule
+                               (relabel, worklist, push, repair) replayed
+                               from one captured hipGraph with its loop
+                               conditions on the device, one host check per
+                               round; schedules it does not cover (learned,
+                               cascade, topology-driven and push-pull
+                               batches, lazy start) run as mode 2           */
   int32_t lazy_start;       /* data/topo batches (stepped drivers): skip the
                                global relabel before the first round when the
                                previous solve ended on fresh labels.  The

@@ -88,11 +88,12 @@ def rmat20(gpu):
     return g, s, t, batches, caps, expect
 
 
-@pytest.mark.parametrize("schedule", ["plain", "learned", "pp", "topo"])
+@pytest.mark.parametrize("schedule", ["plain", "learned", "pp", "topo", "plain-rounds"])
 def test_rmat20_every_batch_exact(gpu, rmat20, schedule):
     dmf = gpu
     g, s, t, batches, caps, expect = rmat20
     p = {"plain": dmf.default_params(g),
+         "plain-rounds": dmf.default_params(g, mode=3),
          "learned": dmf.default_params(g, scheduler=dmf.SCHED_LEARNED),
          "pp": dmf.default_params(g, variant="pp"),
          "topo": dmf.default_params(g, variant="topo")}[schedule]
@@ -136,7 +137,7 @@ def instances256(gpu):
     return cases, caps, expect, solo
 
 
-@pytest.mark.parametrize("mode", [2, 0])
+@pytest.mark.parametrize("mode", [2, 0, 3])
 def test_256_instances_every_batch_exact(gpu, instances256, mode):
     dmf = gpu
     cases, caps, expect, solo = instances256

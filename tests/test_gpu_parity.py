@@ -18,7 +18,7 @@ from util import caps_after, rmat_case
 pytestmark = pytest.mark.gpu
 
 VARIANTS = ["data", "topo", "pp"]
-MODES = [0, 1, 2]  # 0 persistent, 1 stepped, 2 stepped with graph-replayed BFS levels
+MODES = [0, 1, 2, 3]  # 0 persistent, 1 stepped, 2 stepped with graph-replayed BFS levels, 3 round graphs
 
 
 def say(*a):
@@ -614,3 +614,29 @@ def test_cascade_parameter_on_rmat_and_grid(gpu, mode):
                     applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 1500 + b, cap_hi=hi))
                     f = eng.apply_batch(*applied[-1], p)
                     check_engine(eng, g, caps_after(g, applied), s, t, f, f"{sched} batch {b}")
+
+
+@pytest.mark.parametrize("interval", [1, 3])
+def test_round_graphs_on_a_deep_grid(gpu, interval):
+    """Round graphs (mode 3) on a grid whose relabels run deeper than the
+    longest captured round (64 level launches): each such relabel stops the
+    replay, goes on from the host through level segments and ends its round
+    with direct launches; a fresh engine's first relabel does the same with
+    the shortest round.  Exact after the static solve and every batch, with a
+    relabel every round and every third round."""
+    dmf = gpu
+    u, v, w = dmf.gen_grid(40, 40, 8, 1, 1000)
+    g = dmf.build_csr(u, v, w)
+    s, t = 0, g.n - 1
+    p = dmf.default_params(g, mode=3, relabel_interval=interval)
+    say(f"[rounds grid] interval={interval}")
+    with dmf.Engine(g, s, t) as eng:
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "static")
+        assert eng.counters()["global_relabels"] >= 1
+        cur = w.copy()
+        applied = []
+        for b, pct in enumerate((3, 8)):
+            applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 2500 + b, cap_hi=1000))
+            f = eng.apply_batch(*applied[-1], p)
+            check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")

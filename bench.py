@@ -354,6 +354,11 @@ def build_line(args, g, s, t, kc, S, L, variants, roof, cpu, cpu1, verified):
         "learned": pass_object(L) if L is not None else None,
         "plain_variants": {k: pass_object(v) for k, v in variants.items()},
         "learned_speedup_p50": round(S["p50_ms"] / L["p50_ms"], 3) if L is not None else None,
+        # the same ratio against the fixed schedule on the round-graph driver
+        # (which the learned scheduler cannot use: its rounds end on decisions
+        # taken by the host)
+        "learned_speedup_p50_vs_plain_rounds": round(variants["data@rounds"]["p50_ms"] / L["p50_ms"], 3)
+        if L is not None and "data@rounds" in variants else None,
         "flows_and_smin_verified_vs_dinic": verified,
         "roofline": roof,
         "all_kernels": {"achieved_gbs": round(S["achieved_gbs"], 2),
@@ -381,9 +386,10 @@ def main():
     ap.add_argument("--interval", type=int, default=1, help="rounds between global relabels of the plain passes")
     ap.add_argument("--sched", default="both", choices=["both", "plain"],
                     help="time the learned scheduler beside the plain fixed schedule")
-    ap.add_argument("--variants", default="data,pp,topo,data+cascade",
+    ap.add_argument("--variants", default="data,pp,topo,data+cascade,data@rounds",
                     help="variants of the plain fixed-schedule passes (the first is the headline's plain; "
-                         "+cascade adds the engine's cascade sub-rounds)")
+                         "+cascade adds the engine's cascade sub-rounds; @rounds runs the pass on the "
+                         "round-graph driver, mode 3, instead of --mode)")
     ap.add_argument("--cascade", type=int, default=1024,
                     help="cascade sub-round cap of the +cascade passes")
     ap.add_argument("--static-variant", default="pp",
@@ -419,7 +425,10 @@ def main():
     static = dmf.default_params(g, variant=args.static_variant, mode=args.mode)
 
     def plain_params(variant):
-        return dmf.default_params(g, variant=variant, mode=args.mode, relabel_interval=args.interval)
+        # "<variant>@rounds": the same fixed schedule on the round-graph driver
+        base, _, drv = variant.partition("@")
+        mode = 3 if drv == "rounds" else args.mode
+        return dmf.default_params(g, variant=base, mode=mode, relabel_interval=args.interval)
 
     if args.only:
         if args.only == "learned":
@@ -450,7 +459,7 @@ def main():
         # scheduler runs on), to separate them from the scheduler's decisions
         base = var.split("+")[0]
         pp = plain_params(base)
-        if var.endswith("+cascade"):
+        if "+cascade" in var:
             pp.cascade = args.cascade
         recs, fl = run_gpu(g, s, t, batches, pp, args.steps, args.warmup, f"plain {var}",
                            static_params=static, sides=sides if vi == 0 else None)

@@ -370,9 +370,23 @@ int capture_round(dmf_engine* E, int idx, int kc) {
 // check per segment instead of one per level.  The first segment is as long as
 // the previous BFS of this mode was deep; later ones grow.  Same result as
 // relabel_stepped (the same level kernel body runs on the same frontiers).
-int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool early) {
+// list (BFS_SINK): the worklist of the round that follows is built behind the
+// first segment (k_relabel_done, k_cap_after for an early-exit relabel,
+// k_build_wl_after) and taken from the same host check; a deeper BFS runs on
+// and the three are launched again once it is over.
+int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool early, bool list) {
   Dev& D = E->D;
   int rc;
+  const int gn = std::min(blocks_for(E->n), E->grid_cap);
+  auto list_behind = [&]() -> int {
+    hipLaunchKernelGGL(k_relabel_done, dim3(1), dim3(kWave), 0, E->stream, D);
+    if (early) hipLaunchKernelGGL(k_cap_after, dim3(gn), dim3(kBlock), 0, E->stream, D);
+    hipLaunchKernelGGL(k_build_wl_after, dim3(gn), dim3(kBlock), 0, E->stream, D);
+    E->launches += early ? 3 : 2;
+    HIP_OK(hipGetLastError());
+    return DMF_OK;
+  };
+  bool first = true;
   hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev, early ? 1 : 0);
   ++E->launches;
   HIP_OK(hipGetLastError());
@@ -387,6 +401,8 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
     HIP_OK(hipGraphLaunch(E->seg_exec[mode][idx], E->stream));
     if (E->profile) HIP_OK(hipEventRecord(E->pev1, E->stream));
     ++E->launches;
+    if (list && first && (rc = list_behind())) return rc;
+    first = false;
     if ((rc = sync_ctrl(E))) return rc;
     if (E->profile) {
       float ms = 0.f;
@@ -399,6 +415,11 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
     if (lvl > E->n + 1) return DMF_EDEVICE;
     if (idx + 1 < dmf_engine::kSegSizes) ++idx;
   }
+  if (list && E->h_ctrl->bfs_inc) {
+    // the first segment did not finish the BFS: list now
+    if ((rc = list_behind())) return rc;
+    if ((rc = sync_ctrl(E))) return rc;
+  }
   E->seg_levels[mode] = lvl;
   E->last.bfs_levels += lvl;
   E->last.global_relabels += 1;
@@ -409,9 +430,14 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
 
 // early (BFS_SINK only): stop once every vertex active now is labelled; the
 // control block's bfs_cut then tells that the labels are not complete.
-int relabel_run(dmf_engine* E, int mode, int part_cnt, int* visited, bool early) {
+// *listed (may be null): the round's worklist was built behind the relabel
+// (graph-replayed levels from t only); otherwise the caller builds it.
+int relabel_run(dmf_engine* E, int mode, int part_cnt, int* visited, bool early, bool* listed) {
+  const bool list = listed != nullptr && mode == BFS_SINK;
+  if (listed) *listed = false;
   if (E->use_graph && !E->graph_failed) {
-    const int rc = relabel_graph(E, mode, part_cnt, visited, early);
+    const int rc = relabel_graph(E, mode, part_cnt, visited, early, list);
+    if (rc == DMF_OK && list) *listed = true;
     if (rc != kNoGraph) return rc;
     // capture or instantiation refused: keep the host-driven level loop for
     // this engine; relabel_stepped starts the BFS over from its roots
@@ -428,7 +454,8 @@ int relabel_run(dmf_engine* E, int mode, int part_cnt, int* visited, bool early)
 // its caller confirms convergence with an unrestricted relabel).  Relabels
 // from t write the order buffer that does not hold that set, and a complete
 // one (not cut short by an early exit) becomes the set for the next.
-int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = false, bool restrict_prev = false) {
+int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = false, bool restrict_prev = false,
+            bool* listed = nullptr) {
   early = early && mode == BFS_SINK;
   Dev& D = E->D;
   const bool two = mode == BFS_SINK && E->ord[1] != nullptr;
@@ -439,7 +466,7 @@ int relabel(dmf_engine* E, int mode, int part_cnt, int* visited, bool early = fa
     D.nprev = E->full_n;
     D.bu_rest = E->restrict_mode;
   }
-  const int rc = relabel_run(E, mode, part_cnt, visited, early);
+  const int rc = relabel_run(E, mode, part_cnt, visited, early, listed);
   D.prev = nullptr;
   D.nprev = 0;
   // labels of a restricted relabel whose levels all ran top-down are exact
@@ -679,16 +706,19 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
     }
     for (; !rounds;) {
       bool fresh = false;
+      bool listed = false;  // the worklist was built behind this round's relabel
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
         // the relabels that may stop early may also restrict their bottom-up
         // levels to the last complete labelled set
         const bool early = try_early && !need_full;
-        if ((rc = relabel(E, BFS_SINK, 0, &visited, early, early))) return rc;
+        // data-driven rounds build their worklist behind the relabel (one
+        // host check for both) where the levels are graph-replayed
+        if ((rc = relabel(E, BFS_SINK, 0, &visited, early, early, topo ? nullptr : &listed))) return rc;
         since = 0;
         const bool cut = early && E->h_ctrl->bfs_cut;
         fresh = !cut && !E->restricted;
-        if (cut) LAUNCH(k_cap_unlabelled, blocks_for(n), D, std::max(0, E->last_levels - 1));
+        if (cut && !listed) LAUNCH(k_cap_unlabelled, blocks_for(n), D, std::max(0, E->last_levels - 1));
         need_full = false;
       }
       int32_t cnt = 0;
@@ -698,9 +728,11 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         // the list counters were cleared by the last push launch (or k_begin)
         // a labelled set that is a large part of the graph is listed by one
         // coalesced pass over every vertex instead of a gather through order[]
-        const bool all = visited > n / 4;
-        LAUNCH(k_build_wl, blocks_for(all ? n : visited), D, all ? -1 : visited);
-        if ((rc = sync_ctrl(E))) return rc;
+        if (!listed) {
+          const bool all = visited > n / 4;
+          LAUNCH(k_build_wl, blocks_for(all ? n : visited), D, all ? -1 : visited);
+          if ((rc = sync_ctrl(E))) return rc;
+        }
         cnt = E->h_ctrl->wl_cnt;
         wpk = E->h_ctrl->wq[kQActive];
         if (cnt == 0 && wpk == 0) {

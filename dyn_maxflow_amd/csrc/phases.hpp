@@ -82,6 +82,7 @@ struct Ctrl {
   int32_t early;      // the current relabel may stop once its targets are labelled
   int32_t bfs_cut;    // the current relabel stopped before its frontier ran out
   int32_t bfs_rbu;    // the current relabel ran a restricted bottom-up level
+  int32_t bfs_inc;    // k_relabel_done: the relabel's frontier was not empty yet
   int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
                       // read from slot r and written to slot r^1 by each level
   // Round-graph driver: loop state of the data-driven fixed schedule kept on
@@ -2162,6 +2163,43 @@ __global__ void k_relabel_prep(Dev D, Dev* out, int early) {
 __global__ void __launch_bounds__(kBlock) k_cap_unlabelled(Dev D, int cap) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += gridDim.x * blockDim.x)
     if (D.height[v] == D.n && D.excess[v] <= 0) D.height[v] = cap;
+}
+// Worklist behind a relabel's first replayed segment, without a host check in
+// between: k_relabel_done records whether the BFS is over (the frontier at the
+// cursor, slot 0 at a segment's end, is empty) and its labelled count;
+// k_cap_after and k_build_wl_after do nothing while it is not, and the host,
+// seeing bfs_inc, runs further segments and launches the three again.
+__global__ void k_relabel_done(Dev D) {
+  if (threadIdx.x != 0) return;
+  Ctrl* c = D.ctrl;
+  const int lvl = c->bcur[0][1];
+  const int inc = c->qcnt[lvl % 3] != 0 ? 1 : 0;
+  c->bfs_inc = inc;
+  if (!inc) c->visited = c->bcur[0][0];
+}
+// k_cap_unlabelled after an early-exit relabel that stopped short (bfs_cut),
+// with the cap taken from the level cursor.
+__global__ void __launch_bounds__(kBlock) k_cap_after(Dev D) {
+  const Ctrl* c = D.ctrl;
+  if (c->bfs_inc || !c->bfs_cut) return;
+  const int cap = max(0, c->bcur[0][1] - 1);
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += gridDim.x * blockDim.x)
+    if (D.height[v] == D.n && D.excess[v] <= 0) D.height[v] = cap;
+}
+// k_build_wl over the labelled count k_relabel_done recorded (every vertex
+// when that is a quarter of the graph or more).
+__global__ void __launch_bounds__(kBlock) k_build_wl_after(Dev D) {
+  const Ctrl* cc = D.ctrl;
+  if (cc->bfs_inc) return;  // block-uniform
+  const int cnt = cc->visited;
+  if (is_block0_thread0()) {
+    Ctrl* c = D.ctrl;
+    c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+    c->wq[0] = c->wq[1] = c->wq[2] = 0;
+  }
+  Tally t;
+  ph_build_wl(D, make_range(), cnt > D.n / 4 ? -1 : cnt, t);
+  t.flush(D.ctrs);
 }
 // k_flow_prep: list counters, flags, the flow and the per-instance flows,
 // before the flow reduction.

@@ -17,7 +17,7 @@
 //                                  applies the files one after another.
 //   --kc=N                         run only this kernel_cycles value
 //   --relabel-interval=N           rounds between global relabels (default 1)
-//   --driver=graph|stepped|persistent  engine driver (default graph: per-phase
+//   --driver=graph|rounds|stepped|persistent  engine driver (default graph: per-phase
 //                                  launches with the BFS levels replayed from
 //                                  captured hipGraphs); --stepped and
 //                                  --persistent are short forms
@@ -167,6 +167,7 @@ int main(int argc, char** argv) {
     else if (!std::strcmp(x, "--stepped") || !std::strcmp(x, "--driver=stepped")) o.driver = 1;
     else if (!std::strcmp(x, "--persistent") || !std::strcmp(x, "--driver=persistent")) o.driver = 0;
     else if (!std::strcmp(x, "--driver=graph")) o.driver = 2;
+    else if (!std::strcmp(x, "--driver=rounds")) o.driver = 3;
     else if (!std::strcmp(x, "--json")) o.json = true;
     else if (!std::strcmp(x, "--no-static-check")) o.static_check = false;
     else {

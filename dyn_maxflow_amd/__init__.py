@@ -220,6 +220,7 @@ def _lib():
                                                      _I64P]),
         "dmf_device_count": (C.c_int, []),
         "dmf_engine_autotune": (C.c_int, [C.c_void_p, C.POINTER(Params), _I32P, _I32P]),
+        "dmf_engine_autotune_schedule": (C.c_int, [C.c_void_p, C.POINTER(Params), _I32P, _I32P]),
         "dmf_batch_create": (C.c_int, [C.c_int32, C.POINTER(_Csr), _I32P, _I32P, C.POINTER(C.c_void_p)]),
         "dmf_engine_instances": (C.c_int32, [C.c_void_p]),
         "dmf_batch_static_solve": (C.c_int, [C.c_void_p, C.POINTER(Params), _I64P]),
@@ -537,6 +538,14 @@ class Engine:
         a, b = C.c_int32(), C.c_int32()
         _check(_lib().dmf_engine_autotune(self._h, C.byref(params), C.byref(a), C.byref(b)), "autotune")
         return a.value, b.value
+
+    def autotune_schedule(self, params: Params):
+        """autotune, then (fixed schedule) the fastest global-relabel interval, written into
+        params.relabel_interval; returns (lane_max, wide_t, relabel_interval) and resets the engine."""
+        a, b = C.c_int32(), C.c_int32()
+        _check(_lib().dmf_engine_autotune_schedule(self._h, C.byref(params), C.byref(a), C.byref(b)),
+               "autotune_schedule")
+        return a.value, b.value, params.relabel_interval
 
 
 def engine_footprint(n: int, m: int) -> int:

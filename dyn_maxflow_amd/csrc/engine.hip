@@ -2067,6 +2067,34 @@ int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, i
   });
 }
 
+int dmf_engine_autotune_schedule(dmf_engine* E, dmf_params* p, int32_t* lane_max, int32_t* wide_t) {
+  return guarded([&]() -> int {
+    if (!E || !p) return DMF_EINVAL;
+    int rc = dmf_engine_autotune(E, p, lane_max, wide_t);
+    if (rc || p->scheduler != DMF_SCHED_FIXED) return rc;
+    static const int32_t cand[] = {1, 2, 3, 5, 8};
+    double best_ms = 1e300;
+    int32_t best = 1;
+    for (int32_t iv : cand) {
+      dmf_params q = *p;
+      q.relabel_interval = iv;
+      double ms = 0;
+      for (int rep = 0; rep < 2; ++rep) {  // the first solve warms caches
+        if ((rc = dmf_engine_reset(E))) return rc;
+        int64_t f = 0;
+        if ((rc = dmf_engine_static_solve(E, &q, &f))) return rc;
+        ms = E->last.device_ms;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = iv;
+      }
+    }
+    p->relabel_interval = best;
+    return dmf_engine_reset(E);
+  });
+}
+
 int dmf_recalculate_max_flow(const dmf_csr* g, int32_t k, const int32_t* src, const int32_t* dst,
                              const int32_t* w, int32_t s, int32_t t, int32_t kc, int32_t variant,
                              int64_t* flow, double* ms_static, double* ms_batch) {

@@ -4,7 +4,7 @@
 //                   [--ef-hi=32] [--seed=4] [--batches=5] [--pct=5]
 //                   [--repeats=3] [--mode=persistent|stepped|graph|rounds|all, or a comma list]
 //                   [--variant=data|topo|pp] [--sched=plain|learned]
-//                   [--cascade=K] [--no-autotune] [--no-solo] [--json]
+//                   [--cascade=K] [--no-autotune] [--tune-interval] [--no-solo] [--json]
 //
 // Workload (all seeded): instance i draws x uniform in [log_lo, log_hi) and
 // has n_i = floor(2^x) vertices, an edge factor uniform in [ef_lo, ef_hi] and
@@ -16,7 +16,8 @@
 // All instances live in one engine as a disjoint union (dmf_batch_create): a
 // batch of updates for every instance is one dmf_batch_apply, i.e. one
 // cooperative launch in the persistent driver.  Before timing, the degree-bucket
-// thresholds are chosen by dmf_engine_autotune.  The solo baseline solves each
+// thresholds are chosen by dmf_engine_autotune (with --tune-interval, for the fixed
+// schedule, also the relabel interval: dmf_engine_autotune_schedule).  The solo baseline solves each
 // instance in its own engine, one after another, and every instance's flow must
 // equal its solo flow after every batch (a mismatch prints MISMATCH and the
 // exit status is 1).  No reference counterpart: the reference solves one graph
@@ -48,6 +49,7 @@ struct Opts {
   int sched = DMF_SCHED_FIXED;  // of the batched engine (the solo baseline keeps the fixed schedule)
   int cascade = 0;              // cascade sub-rounds per round of the batched engine
   bool autotune = true;
+  bool tune_interval = false;  // fixed schedule: also autotune the relabel interval
   bool solo = true;
   bool json = false;
 };
@@ -122,7 +124,7 @@ double pct_rank(std::vector<double> x, double q) {  // nearest rank
 
 struct ModeResult {
   std::string name;
-  int32_t lane_max = 0, wide_t = 0;
+  int32_t lane_max = 0, wide_t = 0, interval = 1;
   double static_ms = 0;
   std::vector<double> dev_ms, wall_ms;
   uint64_t arc_visits = 0;
@@ -159,6 +161,7 @@ int main(int argc, char** argv) {
     } else if ((a = val("--cascade="))) {
       o.cascade = std::atoi(a);
     } else if (!std::strcmp(x, "--no-autotune")) o.autotune = false;
+    else if (!std::strcmp(x, "--tune-interval")) o.tune_interval = true;
     else if (!std::strcmp(x, "--no-solo")) o.solo = false;
     else if (!std::strcmp(x, "--json")) o.json = true;
     else {
@@ -298,9 +301,12 @@ int main(int argc, char** argv) {
     q.mode = mname == "persistent" ? 0 : mname == "stepped" ? 1 : mname == "graph" ? 2 : 3;
     q.scheduler = o.sched;
     q.cascade = o.cascade;
-    if (o.autotune) {
+    if (o.autotune && o.tune_interval) {
+      if ((rc = dmf_engine_autotune_schedule(E, &q, &R.lane_max, &R.wide_t))) die("autotune_schedule", rc);
+    } else if (o.autotune) {
       if ((rc = dmf_engine_autotune(E, &q, &R.lane_max, &R.wide_t))) die("autotune", rc);
     }
+    R.interval = q.relabel_interval;
     for (int rep = 0; rep < o.repeats + 1; ++rep) {  // repeat 0 warms up
       if ((rc = dmf_engine_reset(E))) die("reset", rc);
       if ((rc = dmf_batch_static_solve(E, &q, flows.data()))) die("batch static solve", rc);
@@ -334,9 +340,9 @@ int main(int argc, char** argv) {
           }
       }
     }
-    std::printf("mode=%s lane_max=%d wide_t=%d static %.3f ms; per batch (all %d instances) p50 %.3f ms "
+    std::printf("mode=%s lane_max=%d wide_t=%d interval=%d static %.3f ms; per batch (all %d instances) p50 %.3f ms "
                 "p99 %.3f ms wall p50 %.3f ms; GTEPS %.3f; launches/batch %.1f\n",
-                mname.c_str(), R.lane_max, R.wide_t, R.static_ms, o.instances, pct_rank(R.dev_ms, 50),
+                mname.c_str(), R.lane_max, R.wide_t, R.interval, R.static_ms, o.instances, pct_rank(R.dev_ms, 50),
                 pct_rank(R.dev_ms, 99), pct_rank(R.wall_ms, 50), R.arc_visits / (R.arc_ms * 1e6),
                 R.dev_ms.empty() ? 0.0 : static_cast<double>(R.launches) / R.dev_ms.size());
     std::fflush(stdout);
@@ -358,9 +364,9 @@ int main(int argc, char** argv) {
     for (size_t r = 0; r < results.size(); ++r) {
       const ModeResult& R = results[r];
       const double p50 = pct_rank(R.dev_ms, 50);
-      std::printf("%s{\"mode\":\"%s\",\"lane_max\":%d,\"wide_t\":%d,\"static_ms\":%.4f,\"batch_p50_ms\":%.4f,"
-                  "\"batch_p99_ms\":%.4f,\"wall_p50_ms\":%.4f,\"gteps\":%.4f,\"speedup_vs_solo\":%.3f}",
-                  r ? "," : "", R.name.c_str(), R.lane_max, R.wide_t, R.static_ms, p50, pct_rank(R.dev_ms, 99),
+      std::printf("%s{\"mode\":\"%s\",\"lane_max\":%d,\"wide_t\":%d,\"relabel_interval\":%d,\"static_ms\":%.4f,"
+                  "\"batch_p50_ms\":%.4f,\"batch_p99_ms\":%.4f,\"wall_p50_ms\":%.4f,\"gteps\":%.4f,\"speedup_vs_solo\":%.3f}",
+                  r ? "," : "", R.name.c_str(), R.lane_max, R.wide_t, R.interval, R.static_ms, p50, pct_rank(R.dev_ms, 99),
                   pct_rank(R.wall_ms, 50), R.arc_visits / (R.arc_ms * 1e6), p50 > 0 ? solo_p50 / p50 : 0.0);
     }
     std::printf("]}\n");

@@ -300,6 +300,14 @@ int dmf_engine_reserve_updates(dmf_engine* e, int32_t k);
  * vertex). */
 int dmf_engine_autotune(dmf_engine* e, const dmf_params* p, int32_t* lane_max, int32_t* wide_t);
 
+/* dmf_engine_autotune, then, for the fixed schedule (p->scheduler ==
+ * DMF_SCHED_FIXED), the global-relabel interval: a static solve from reset is
+ * timed under each interval in {1, 2, 3, 5, 8} with the chosen thresholds and
+ * the fastest is written to p->relabel_interval (the learned scheduler times
+ * its own relabels and keeps p unchanged).  Leaves the engine reset.  No
+ * reference counterpart (the reference relabels every round). */
+int dmf_engine_autotune_schedule(dmf_engine* e, dmf_params* p, int32_t* lane_max, int32_t* wide_t);
+
 /* ---------------------------------------------------- batched instances */
 /* Many independent (graph, s, t) instances kept resident as one disjoint union
  * and solved together: every phase of a solve covers all instances at once

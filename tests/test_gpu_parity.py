@@ -640,3 +640,25 @@ def test_round_graphs_on_a_deep_grid(gpu, interval):
             applied.append(dmf.gen_updates(u, v, cur, len(u) * pct // 100, 2500 + b, cap_hi=1000))
             f = eng.apply_batch(*applied[-1], p)
             check_engine(eng, g, caps_after(g, applied), s, t, f, f"batch {b}")
+
+
+def test_autotuned_schedule_keeps_results(gpu):
+    """dmf_engine_autotune_schedule picks thresholds and, for the fixed
+    schedule, a relabel interval among its candidates; results stay exact
+    under the tuned parameters, and the learned scheduler's are unchanged."""
+    dmf = gpu
+    g, s, t, (u, v, w) = rmat_case(12, 16, 12)
+    with dmf.Engine(g, s, t) as eng:
+        p = dmf.default_params(g)
+        lm, wt, iv = eng.autotune_schedule(p)
+        assert lm > 0 and wt > 0 and iv in (1, 2, 3, 5, 8) and p.relabel_interval == iv
+        say(f"[autotune schedule] lane_max={lm} wide_t={wt} interval={iv}")
+        f = eng.static_solve(p)
+        check_engine(eng, g, g.cap, s, t, f, "tuned static")
+        cur = w.copy()
+        src, dst, nw = dmf.gen_updates(u, v, cur, len(u) // 25, 78)
+        f = eng.apply_batch(src, dst, nw, p)
+        check_engine(eng, g, caps_after(g, [(src, dst, nw)]), s, t, f, "tuned batch")
+        q = dmf.default_params(g, scheduler=dmf.SCHED_LEARNED)
+        eng.autotune_schedule(q)
+        assert q.relabel_interval == 1

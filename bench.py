@@ -273,6 +273,8 @@ def roofline(prof, mode):
     bytes_alg = BYTES_ARC * prof["bfs_arc_visits"] + BYTES_VERTEX * prof["bfs_vertex_visits"]
     gbs = bytes_alg / (ms / 1e3) / 1e9
     tr = traffic_from_profile()
+    if tr and tr.get("kernel") != LEVEL_KERNEL[mode]:
+        tr = None  # the committed counter summary is of another driver's level kernel
     return {
         "bound": "hbm",
         "kernel": LEVEL_KERNEL[mode],

@@ -24,6 +24,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <climits>
+#include <cstddef>
 #include <cstdint>
 
 #include "../../include/dmf.h"
@@ -105,6 +106,9 @@ struct Ctrl {
   unsigned long long r_wpk;      // round-graph driver: this round's wide queue word
   long long flow;
 };
+// k_bfs_level_dev loads the ring counts and wide words as vectors
+static_assert(offsetof(Ctrl, qcnt) % 16 == 0 && offsetof(Ctrl, wq) % 16 == 0 && offsetof(Ctrl, bcur) % 8 == 0,
+              "control-block fields read as vectors must stay aligned");
 
 struct Dev {
   int32_t n, m, s, t;  // s, t: the first source and sink (feature of the scheduler)
@@ -2172,10 +2176,13 @@ __global__ void __launch_bounds__(kBlock) k_cap_unlabelled(Dev D, int cap) {
 __global__ void k_relabel_done(Dev D) {
   if (threadIdx.x != 0) return;
   Ctrl* c = D.ctrl;
-  const int lvl = c->bcur[0][1];
-  const int inc = c->qcnt[lvl % 3] != 0 ? 1 : 0;
+  // the ring counts are loaded beside the cursor (one round trip)
+  const int2 cur = *reinterpret_cast<const int2*>(&c->bcur[0][0]);
+  const int4 qc = *reinterpret_cast<const int4*>(&c->qcnt[0]);
+  const int slot = cur.y % 3;
+  const int inc = (slot == 0 ? qc.x : (slot == 1 ? qc.y : qc.z)) != 0 ? 1 : 0;
   c->bfs_inc = inc;
-  if (!inc) c->visited = c->bcur[0][0];
+  if (!inc) c->visited = cur.x;
 }
 // k_cap_unlabelled after an early-exit relabel that stopped short (bfs_cut),
 // with the cap taken from the level cursor.
@@ -2278,27 +2285,38 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level(Dev D, int mode, int lo, i
 // during this launch, and the previous launch's writes are visible at the
 // launch boundary.  Uncached loads of the same words by every wave of a large
 // grid would serialise at one memory channel.
-__global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, int mode, int r) {
-  const Dev D = *pD;
-  const Ctrl* cc = D.ctrl;
-  const int lo = cc->bcur[r][0];
-  const int lvl = cc->bcur[r][1];
-  const int cnt = cc->qcnt[lvl % 3];
-  const unsigned long long wpk = cc->wq[lvl % 3];
+// The control block comes in as an argument (its address is fixed for the
+// engine's life) and the three ring counts and wide words are loaded whole,
+// next to the cursor, so a launch learns whether its level is empty after one
+// memory round trip instead of three (Dev copy, cursor, ring slot); the Dev
+// copy is read only by launches that have a frontier.  Empty trailing levels
+// are a third of the launches of a replayed segment.
+__global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict__ pD, Ctrl* ctrl, int mode, int r) {
+  const int2 cur = *reinterpret_cast<const int2*>(&ctrl->bcur[r][0]);
+  const int4 qc = *reinterpret_cast<const int4*>(&ctrl->qcnt[0]);  // qcnt[0..2], wl_cnt
+  const ulonglong2 w01 = *reinterpret_cast<const ulonglong2*>(&ctrl->wq[0]);
+  const unsigned long long w2 = ctrl->wq[2];
+  const int early_flag = ctrl->early;
+  const int lo = cur.x;
+  const int lvl = cur.y;
+  const int slot = lvl % 3;
+  const int cnt = slot == 0 ? qc.x : (slot == 1 ? qc.y : qc.z);
+  const unsigned long long wpk = slot == 0 ? w01.x : (slot == 1 ? w01.y : w2);
   if (is_block0_thread0()) {
-    D.ctrl->bcur[r ^ 1][0] = lo + cnt;
-    D.ctrl->bcur[r ^ 1][1] = cnt ? lvl + 1 : lvl;
+    ctrl->bcur[r ^ 1][0] = lo + cnt;
+    ctrl->bcur[r ^ 1][1] = cnt ? lvl + 1 : lvl;
     if (cnt) {
-      D.ctrl->qcnt[(lvl + 2) % 3] = 0;
-      D.ctrl->wq[(lvl + 2) % 3] = 0;
+      ctrl->qcnt[(lvl + 2) % 3] = 0;
+      ctrl->wq[(lvl + 2) % 3] = 0;
     }
   }
   if (cnt == 0) return;  // uniform: the BFS is over
+  const Dev D = *pD;
   // early-exit relabel whose targets are all labelled: this level is not
   // expanded, so the next one comes out empty.  A block that starts late may
   // see counts that blocks still running have raised and stop while others
   // expand; a partly expanded level still labels exact distances.
-  const bool early = cc->early != 0;
+  const bool early = early_flag != 0;
   if (early && block_targets_labelled(D)) {
     if (threadIdx.x == 0) D.ctrl->bfs_cut = 1;
     return;

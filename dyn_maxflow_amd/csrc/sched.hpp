@@ -167,6 +167,10 @@ struct SchedRound {
 // it with the round's tag.
 __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist,
                                                     const SchedWeights* W, SchedState* S, SchedRound a) {
+  // block 0 loads the decision tag beside the list sizes (one round trip for
+  // the empty launches that end every chain)
+  const bool decider = W != nullptr && blockIdx.x == 0;
+  const int32_t tag_now = decider ? ldr(&S->tag) : a.tag;
   const Ctrl* cc = D.ctrl;
   const int cnt = cc->qcnt[slot];
   const unsigned long long wpk = cc->wq[slot];
@@ -176,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int
   }
   if (cnt == 0 && wpk == 0) {
     // uniform over block 0: the tag was written by an earlier launch
-    if (W != nullptr && blockIdx.x == 0 && ldr(&S->tag) != a.tag) {
+    if (decider && tag_now != a.tag) {
       __shared__ SchedWeights wl;
       const float* src = reinterpret_cast<const float*>(W);
       float* dst = reinterpret_cast<float*>(&wl);

@@ -17,7 +17,7 @@
 //                              contiguous slices of it)
 //   wl[n] wl2[n] part[n]       worklists (active / pull / stranded part)
 //   side[n] pfinal[n] (u8)     push-pull partition (d_partB, d_partfinal)
-//   hkey/hval                  open-addressing (tail,head) -> arc index, so an
+//   htab (16-byte slots)       open-addressing (tail,head) -> arc index, so an
 //                              update finds its arc in O(1) instead of the
 //                              reference's scan of the tail's adjacency
 //   wq_vert/wq_base            wide-vertex chunk queues; wide_vert/wide_base
@@ -653,6 +653,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
   const bool pp = C.variant == DMF_VARIANT_PP;
   const bool topo = C.variant == DMF_VARIANT_TOPO;
   if (C.entry == 1) {
+    // the apply releases the arcs it updates (no release pass)
     LAUNCH(k_upd_apply, blocks_for(std::max<int64_t>(D.k, D.wide_packed >> 32)), D, 0, pp ? 1 : 0);
     if (pp) {
       LAUNCH(k_pp_cut, phase_blocks(n, chunks_of(D.wide_packed)), D);
@@ -660,7 +661,6 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       LAUNCH(k_upd_clamp, blocks_for(D.k), D);
       LAUNCH(k_saturate_source, blocks_for(std::min<int64_t>(E->m, 1 << 20)), D);
     }
-    LAUNCH(k_upd_release, blocks_for(D.k), D);
   }
   int round = 0;
   bool conv = true;
@@ -1356,7 +1356,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   // failed allocation part way through frees what was already allocated.
   int32_t *row = nullptr, *col = nullptr, *rev = nullptr, *cap = nullptr, *res = nullptr,
           *rres = nullptr, *height = nullptr, *order = nullptr, *wl = nullptr, *wl2 = nullptr,
-          *part = nullptr, *owner = nullptr, *hval = nullptr;
+          *part = nullptr, *owner = nullptr;
   int32_t *d_srcs = nullptr, *d_snks = nullptr, *d_inst = nullptr;
   uint8_t* d_term = nullptr;
   long long* iflow = nullptr;
@@ -1364,7 +1364,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
   long long* excess = nullptr;
   uint8_t *side = nullptr, *pfinal = nullptr;
   Ctrl* ctrl = nullptr;
-  unsigned long long* hkey = nullptr;
+  ulonglong2* htab = nullptr;
   uint16_t* lev16 = nullptr;
   uint8_t* tgt = nullptr;
   int32_t* cmark = nullptr;
@@ -1375,7 +1375,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       (rc = dalloc(E, &rres, m)) || (rc = dalloc(E, &excess, n)) || (rc = dalloc(E, &height, n)) ||
       (rc = dalloc(E, &order, n)) || (rc = dalloc(E, &wl, n)) || (rc = dalloc(E, &wl2, n)) ||
       (rc = dalloc(E, &part, n)) || (rc = dalloc(E, &side, n)) || (rc = dalloc(E, &pfinal, n)) ||
-      (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &hkey, hcap)) || (rc = dalloc(E, &hval, hcap)) ||
+      (rc = dalloc(E, &ctrl, 1)) || (rc = dalloc(E, &htab, hcap)) ||
       (rc = dalloc(E, &owner, m)) || (rc = dalloc(E, &ctrs, static_cast<size_t>(C_N) * kSlots)) ||
       (rc = dalloc(E, &d_term, n)) || (rc = dalloc(E, &d_srcs, srcs.size())) ||
       (rc = dalloc(E, &d_snks, snks.size())) || (rc = dalloc(E, &iflow, static_cast<size_t>(ninst))) ||
@@ -1384,7 +1384,7 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       (!E->h_vmap.empty() && (rc = dalloc(E, &E->d_vmap, n)));
   D.row = row; D.col = col; D.rev = rev; D.cap = cap; D.res = res; D.rres = rres;
   D.excess = excess; D.height = height; D.order = order; D.wl = wl; D.wl2 = wl2; D.part = part;
-  D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.hkey = hkey; D.hval = hval;
+  D.side = side; D.pfinal = pfinal; D.ctrl = ctrl; D.htab = htab;
   D.hmask = hcap - 1; D.owner = owner; D.ctrs = ctrs;
   D.term = d_term; D.srcs = d_srcs; D.snks = d_snks; D.inst = d_inst; D.iflow = iflow;
   D.lev16 = lev16;
@@ -1438,9 +1438,8 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
             hipMemsetAsync(cmark, 0, sizeof(int32_t) * n, E->stream) == hipSuccess &&
             hipMemsetAsync(pfinal, 0, n, E->stream) == hipSuccess &&
             hipMemsetAsync(ctrl, 0, sizeof(Ctrl), E->stream) == hipSuccess &&
-            hipMemsetAsync(hkey, 0xff, sizeof(unsigned long long) * hcap, E->stream) == hipSuccess &&
-            hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess &&
-            hipMemsetAsync(hval, 0xff, sizeof(int32_t) * hcap, E->stream) == hipSuccess;
+            hipMemsetAsync(htab, 0xff, sizeof(ulonglong2) * hcap, E->stream) == hipSuccess &&
+            hipMemsetAsync(owner, 0xff, sizeof(int32_t) * m, E->stream) == hipSuccess;
   if (!ok || setup_wide(E, kLaneMax, kWideT) != DMF_OK) {
     dmf_engine_destroy(E);
     return DMF_EDEVICE;
@@ -1726,8 +1725,7 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(D.side);
     hipFree(D.pfinal);
     hipFree(D.ctrl);
-    hipFree(D.hkey);
-    hipFree(D.hval);
+    hipFree(D.htab);
     hipFree(D.owner);
     hipFree(D.ctrs);
     hipFree(const_cast<uint8_t*>(D.term));
@@ -1973,7 +1971,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += N;                                       // early-exit relabel targets
   b += 4 * N;                                   // cascade list stamps
   b += 4 * N;                                   // second BFS order buffer
-  b += static_cast<int64_t>(hcap) * 12;         // arc hash (key, value)
+  b += static_cast<int64_t>(hcap) * 16;         // arc hash slots {key, value}
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)
   b += 3 * 16 * (M / kChunk + M / 64 + 1);      // BFS ring chunk tables: chunks of the vertices above wide_t
@@ -2248,8 +2246,7 @@ int dmf_static_recalculate_noalloc(const dmf_csr* g, int32_t k, const int32_t* s
     int32_t* snks = na_at<int32_t>(base, L.snks);
     D.srcs = srcs;
     D.snks = snks;
-    D.hkey = nullptr;  // arcs are found by scanning the tail's adjacency
-    D.hval = nullptr;
+    D.htab = nullptr;  // arcs are found by scanning the tail's adjacency
     D.hmask = 0;
     int32_t* usrc = na_at<int32_t>(base, L.usrc);
     int32_t* udst = na_at<int32_t>(base, L.udst);

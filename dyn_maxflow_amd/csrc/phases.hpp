@@ -158,8 +158,9 @@ struct Dev {
   const int32_t* prev;
   int32_t nprev;
   int32_t bu_rest;
-  unsigned long long* hkey;
-  int32_t* hval;
+  // (tail, head) -> arc index: open-addressing slots {key, arc index in the
+  // low 32 bits of y}, so a probe reads key and value in one 16-byte load
+  ulonglong2* htab;
   unsigned long long hmask;
   int32_t* owner;
   // wide queues (kNumQ x wq_cap) and the static list of every wide vertex
@@ -2028,7 +2029,7 @@ __device__ __forceinline__ unsigned long long pair_key(int u, int v) {
 }
 
 __device__ int lookup_arc(const Dev& D, int u, int v) {
-  if (!D.hkey) {
+  if (!D.htab) {
     // no arc index (the allocation-free solve): scan u's arcs, the last match
     // wins as in OnAdd_kernel's scan (code/dyn_data.h:191-201)
     int found = -1;
@@ -2039,9 +2040,9 @@ __device__ int lookup_arc(const Dev& D, int u, int v) {
   const unsigned long long key = pair_key(u, v);
   unsigned long long h = hash_key(key) & D.hmask;
   for (;;) {
-    const unsigned long long k = D.hkey[h];
-    if (k == key) return D.hval[h];
-    if (k == kEmptyKey) return -1;
+    const ulonglong2 slot = D.htab[h];
+    if (slot.x == key) return static_cast<int>(static_cast<unsigned>(slot.y));
+    if (slot.x == kEmptyKey) return -1;
     h = (h + 1) & D.hmask;
   }
 }
@@ -2075,11 +2076,20 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
 // arc: residual and its mirror move with the capacity.  capacity_only mirrors
 // update_kernel (code/dyn_data_bench.cu:300-322).  zero_wide clears the excess
 // of every wide vertex for the push-pull recompute that follows.
+// The winning record also releases its arc (owner back to -1) and every
+// record that lost its arc is marked in uarc (-1), so the clamp that follows
+// needs no owner reads and no release pass runs after a successful apply.  A
+// losing record reads owner as the winner's index or, once released, -1:
+// neither equals its own index.
 __device__ void ph_upd_apply(const Dev& D, const Range& r, bool capacity_only, bool zero_wide) {
   if (D.ctrl->missing || D.ctrl->invalid) return;
   for (int i = r.gid; i < D.k; i += r.gsz) {
     const int arc = D.uarc[i];
-    if (arc < 0 || D.owner[arc] != i) continue;
+    if (arc < 0) continue;
+    if (D.owner[arc] != i) {
+      D.uarc[i] = -1;
+      continue;
+    }
     const int nw = D.uw[i];
     if (!capacity_only) {
       const int delta = nw - D.cap[arc];
@@ -2087,6 +2097,7 @@ __device__ void ph_upd_apply(const Dev& D, const Range& r, bool capacity_only, b
       D.rres[D.rev[arc]] += delta;
     }
     D.cap[arc] = nw;
+    D.owner[arc] = -1;
   }
   if (zero_wide) {
     const int slots = static_cast<int>(D.wide_packed >> 32);
@@ -2103,7 +2114,7 @@ __device__ void ph_upd_clamp(const Dev& D, const Range& r, Tally& t) {
   if (D.ctrl->missing || D.ctrl->invalid) return;
   for (int i = r.gid; i < D.k; i += r.gsz) {
     const int arc = D.uarc[i];
-    if (arc < 0 || D.owner[arc] != i) continue;
+    if (arc < 0) continue;  // the apply marked every record that lost its arc
     const int rs = D.res[arc];
     t.arc += 1;
     if (rs < 0) {
@@ -2596,9 +2607,10 @@ __global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
     const unsigned long long key = pair_key(a, D.col[e]);
     unsigned long long h = hash_key(key) & D.hmask;
     for (;;) {
-      const unsigned long long prev = atomicCAS(&D.hkey[h], kEmptyKey, key);
+      const unsigned long long prev = atomicCAS(&D.htab[h].x, kEmptyKey, key);
       if (prev == kEmptyKey || prev == key) {
-        atomicMax(&D.hval[h], e);
+        // the value word starts all ones (-1 in its low half, little-endian)
+        atomicMax(reinterpret_cast<int*>(&D.htab[h].y), e);
         break;
       }
       h = (h + 1) & D.hmask;
@@ -2689,8 +2701,6 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(Dev D, Cfg C) {
       G.sync();
       ph_saturate_source(D, r, t);
     }
-    G.sync();
-    ph_upd_release(D, r);
     if (is_block0_thread0()) ctr_add(D.ctrs, C_UPD, static_cast<unsigned long long>(D.k));
     G.sync();
   }

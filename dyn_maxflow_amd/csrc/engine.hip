@@ -1467,10 +1467,26 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       return DMF_EDEVICE;
     }
   }
-  hipLaunchKernelGGL(k_build_hash, dim3(blocks_for(m)), dim3(kBlock), 0, E->stream, D);
-  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E->stream) != hipSuccess) {
-    dmf_engine_destroy(E);
-    return DMF_EDEVICE;
+  {
+    // the arc index is keyed by caller ids: a scattered engine builds it
+    // through the inverse vertex map, held on the device for the build only
+    int32_t* d_ivmap = nullptr;
+    bool ok = true;
+    if (!E->h_vmap.empty()) {
+      std::vector<int32_t> iv(static_cast<size_t>(n));
+      for (int32_t v = 0; v < n; ++v) iv[E->h_vmap[v]] = v;
+      ok = hipMalloc(reinterpret_cast<void**>(&d_ivmap), sizeof(int32_t) * n) == hipSuccess &&
+           hipMemcpy(d_ivmap, iv.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (ok) {
+      hipLaunchKernelGGL(k_build_hash, dim3(blocks_for(m)), dim3(kBlock), 0, E->stream, D, d_ivmap);
+      ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(E->stream) == hipSuccess;
+    }
+    if (d_ivmap) hipFree(d_ivmap);
+    if (!ok) {
+      dmf_engine_destroy(E);
+      return DMF_EDEVICE;
+    }
   }
   // Stepped grids: kGridBlocksPerCu blocks per CU (DMF_GRID_BLOCKS_PER_CU).
   // Cooperative grid: as many blocks as can be co-resident.

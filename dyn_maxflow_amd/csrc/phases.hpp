@@ -2028,16 +2028,21 @@ __device__ __forceinline__ unsigned long long pair_key(int u, int v) {
          static_cast<unsigned>(v);
 }
 
-__device__ int lookup_arc(const Dev& D, int u, int v) {
+// Arc (a, b) in the caller's vertex numbering.  The hash index is keyed by
+// caller ids (k_build_hash maps every arc's ends back through the inverse of
+// vmap), so a probe needs no vmap gathers.
+__device__ int lookup_arc(const Dev& D, int a, int b) {
   if (!D.htab) {
-    // no arc index (the allocation-free solve): scan u's arcs, the last match
-    // wins as in OnAdd_kernel's scan (code/dyn_data.h:191-201)
+    // no arc index (the allocation-free solve): scan the tail's arcs, the
+    // last match wins as in OnAdd_kernel's scan (code/dyn_data.h:191-201)
+    const int u = D.vmap ? D.vmap[a] : a;
+    const int v = D.vmap ? D.vmap[b] : b;
     int found = -1;
     for (int e = D.row[u]; e < D.row[u + 1]; ++e)
       if (D.col[e] == v) found = e;
     return found;
   }
-  const unsigned long long key = pair_key(u, v);
+  const unsigned long long key = pair_key(a, b);
   unsigned long long h = hash_key(key) & D.hmask;
   for (;;) {
     const ulonglong2 slot = D.htab[h];
@@ -2057,7 +2062,7 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
     if (a < 0 || a >= D.n || b < 0 || b >= D.n || D.uw[i] < 0) {  // capacities are >= 0
       ++invalid;
     } else if (a != b) {  // self-loops skipped (code/update.hpp:82, code/dyn_data.h:194)
-      arc = D.vmap ? lookup_arc(D, D.vmap[a], D.vmap[b]) : lookup_arc(D, a, b);
+      arc = lookup_arc(D, a, b);
       if (arc < 0) {
         ++missing;
       } else {
@@ -2596,7 +2601,9 @@ __global__ void __launch_bounds__(kBlock) k_build_static_lists(Dev D, int32_t* w
 
 // Builds the (tail, head) -> arc hash index, one thread per arc; for a repeated
 // (tail, head) pair the highest arc index wins, as the reference's last-match scan.
-__global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
+// ivmap (engine id -> caller id; null when the engine keeps the caller's
+// numbering) keys the index by the ids update records carry.
+__global__ void __launch_bounds__(kBlock) k_build_hash(Dev D, const int32_t* __restrict__ ivmap) {
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < D.m; e += gridDim.x * blockDim.x) {
     int a = 0, b = D.n - 1;  // tail = largest v with row[v] <= e
     while (a < b) {
@@ -2604,7 +2611,8 @@ __global__ void __launch_bounds__(kBlock) k_build_hash(Dev D) {
       if (D.row[mid] <= e) a = mid;
       else b = mid - 1;
     }
-    const unsigned long long key = pair_key(a, D.col[e]);
+    const int head = D.col[e];
+    const unsigned long long key = ivmap ? pair_key(ivmap[a], ivmap[head]) : pair_key(a, head);
     unsigned long long h = hash_key(key) & D.hmask;
     for (;;) {
       const unsigned long long prev = atomicCAS(&D.htab[h].x, kEmptyKey, key);

@@ -115,12 +115,16 @@ def test_reference_entry_points_agree(gpu, golden_dir, expected, variant):
     assert ms_s > 0 and ms_b > 0 and ms > 0
 
 
+@pytest.mark.parametrize("scatter", ["0", "1"])
 @pytest.mark.parametrize("mode", MODES)
-def test_batch_edge_cases(gpu, mode):
+def test_batch_edge_cases(gpu, mode, scatter, monkeypatch):
+    """Update records through the arc index (keyed by the caller's ids) with
+    the engine numbering scattered and not."""
     dmf = gpu
+    monkeypatch.setenv("DMF_VERTEX_SCATTER", scatter)
     g, s, t, (u, v, w) = rmat_case(10, 8, 3)
     p = dmf.default_params(g, mode=mode)
-    say(f"[edge cases] mode={mode}")
+    say(f"[edge cases] mode={mode} scatter={scatter}")
     with dmf.Engine(g, s, t) as eng:
         f0 = eng.static_solve(p)
         # empty batch

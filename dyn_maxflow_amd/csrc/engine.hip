@@ -68,6 +68,9 @@ struct dmf_engine {
   // short-lived blocks is limited by how fast blocks are dispatched and
   // retired, not by the work in them.
   int grid_cap = 8192;
+  // Push, repair and cascade launches (random-access discharge loops whose
+  // waves wait on gathers and atomics) take a few more blocks per CU.
+  int push_cap = 8192;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   dmf_counters last{};
   int64_t bytes = 0;
@@ -174,6 +177,9 @@ int dalloc(dmf_engine* E, T** p, size_t count) {
 }
 
 constexpr int kGridBlocksPerCu = 4;
+// Push/repair/cascade grids: 6 blocks per CU measured 2-3% faster per learned
+// RMAT-20 batch than 4 (8 the same as 6); the BFS levels stay at 4.
+constexpr int kPushBlocksPerCu = 6;
 // Cascade sub-rounds launched between host checks, and the learned
 // scheduler's cap on them per round when the parameters give none.
 constexpr int kCascadeChunk = 8;
@@ -200,6 +206,15 @@ int sync_ctrl(dmf_engine* E) {
     HIP_OK(hipGetLastError());                                                        \
   } while (0)
 #define LAUNCH(kernel, blocks, ...) LAUNCH_ON(E->stream, kernel, blocks, __VA_ARGS__)
+// push, repair and cascade launches: capped at push_cap instead of grid_cap
+#define LAUNCH_PUSH_ON(st, kernel, blocks, ...)                                       \
+  do {                                                                                \
+    hipLaunchKernelGGL(kernel, dim3(std::min<int>((blocks), E->push_cap)), dim3(kBlock), 0, (st), \
+                       __VA_ARGS__);                                                  \
+    ++E->launches;                                                                    \
+    HIP_OK(hipGetLastError());                                                        \
+  } while (0)
+#define LAUNCH_PUSH(kernel, blocks, ...) LAUNCH_PUSH_ON(E->stream, kernel, blocks, __VA_ARGS__)
 
 int zero_dev(dmf_engine* E, void* p, size_t bytes) {
   HIP_OK(hipMemsetAsync(p, 0, bytes, E->stream));
@@ -750,8 +765,8 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       }
       if (topo) {
         const int fs = round & 1;
-        LAUNCH(k_push, phase_blocks(n, static_chunks), D, 0, n, C.kc, PM_ALL, 0ull, fs);
-        LAUNCH(k_repair, phase_blocks(n, static_chunks), D, 0, n, RM_ALL, 0ull);
+        LAUNCH_PUSH(k_push, phase_blocks(n, static_chunks), D, 0, n, C.kc, PM_ALL, 0ull, fs);
+        LAUNCH_PUSH(k_repair, phase_blocks(n, static_chunks), D, 0, n, RM_ALL, 0ull);
         if ((rc = sync_ctrl(E))) return rc;
         if (!E->h_ctrl->flag[fs]) break;
         active = static_cast<double>(n);
@@ -761,19 +776,19 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
           if ((rc = zero_dev(E, D.cmark, sizeof(int32_t) * static_cast<size_t>(n)))) return rc;
           E->cstamp = 0;
         }
-        LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
+        LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
                sub ? ++E->cstamp : 0, E->cascade_relist);
-        LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
+        LAUNCH_PUSH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
         active = static_cast<double>(cnt) + static_cast<double>(wpk >> 32);
         // cascade sub-rounds: each pushes from the list the pass before it
         // wrote, launched kCascadeChunk at a time with one host check between
         // chunks (an empty list ends the chain)
         const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
         if (sub > 0) {
-          // the stepped grid (4 blocks per CU): the first sub-rounds carry
-          // lists as long as the round's own; 2 blocks per CU measured 2-3%
-          // slower per learned batch on RMAT-20, 6 and 8 the same as 4
-          const int cgrid = std::max(1, E->grid_cap);
+          // the push grid: the first sub-rounds carry lists as long as the
+          // round's own; 2 blocks per CU measured 2-3% slower per learned
+          // batch on RMAT-20 than 4
+          const int cgrid = std::max(1, E->push_cap);
           // the scheduler's round-end decision rides on the first launch
           // that finds the chain's list empty
           const SchedWeights* sw = consult ? E->d_sched_w : nullptr;
@@ -782,7 +797,7 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
           for (int done_sub = 0; done_sub < sub;) {
             const int chunk = std::min(kCascadeChunk, sub - done_sub);
             for (int j = 0; j < chunk; ++j) {
-              LAUNCH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist, sw, E->d_sched, sr);
+              LAUNCH_PUSH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist, sw, E->d_sched, sr);
               slot = (slot + 1) % 3;
             }
             done_sub += chunk;
@@ -854,17 +869,17 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         // back into the engine's stream.
         HIP_OK(hipEventRecord(E->ev_fork, E->stream));
         HIP_OK(hipStreamWaitEvent(E->pull_stream, E->ev_fork, 0));
-        LAUNCH_ON(E->pull_stream, k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
-        LAUNCH_ON(E->pull_stream, k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
+        LAUNCH_PUSH_ON(E->pull_stream, k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+        LAUNCH_PUSH_ON(E->pull_stream, k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
         HIP_OK(hipEventRecord(E->ev_join, E->pull_stream));
-        LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
-        LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
+        LAUNCH_PUSH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
+        LAUNCH_PUSH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
         HIP_OK(hipStreamWaitEvent(E->stream, E->ev_join, 0));
       } else {
-        if (side_b) LAUNCH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
-        if (side_a) LAUNCH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
-        if (side_b) LAUNCH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
-        if (side_a) LAUNCH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
+        if (side_b) LAUNCH_PUSH(k_push, phase_blocks(np, chunks_of(wp)), D, 1, np, C.kc, PM_ONLY_B, wp, 0);
+        if (side_a) LAUNCH_PUSH(k_push, phase_blocks(nq, chunks_of(wq)), D, 2, nq, C.kc, PM_ONLY_A, wq, 0);
+        if (side_b) LAUNCH_PUSH(k_repair, phase_blocks(np, chunks_of(wp)), D, 1, np, RM_B, wp);
+        if (side_a) LAUNCH_PUSH(k_repair, phase_blocks(nq, chunks_of(wq)), D, 2, nq, RM_PULL_A, wq);
       }
       active = static_cast<double>(np) + nq + static_cast<double>(wp >> 32) + static_cast<double>(wq >> 32);
       ++round;
@@ -892,8 +907,8 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
             conv = false;
             break;
           }
-          LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0);
-          LAUNCH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_PFINAL, wpk);
+          LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0);
+          LAUNCH_PUSH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_PFINAL, wpk);
           ++round;
         }
         if (conv) LAUNCH(k_find_final_cut, blocks_for(n), D);
@@ -1057,8 +1072,8 @@ int complete_phase(dmf_engine* E, bool deficits, int kc, int max_rounds) {
     const unsigned long long wpk = E->h_ctrl->wq[deficits ? kQPull : kQActive];
     if (cnt == 0 && wpk == 0) return DMF_OK;
     if (round >= max_rounds) return DMF_ENOCONV;
-    if (deficits) LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 2, cnt, kc, PM_NO_SRC, wpk, 0);
-    else LAUNCH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, kc, PM_NO_SNK, wpk, 0);
+    if (deficits) LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 2, cnt, kc, PM_NO_SRC, wpk, 0);
+    else LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, kc, PM_NO_SNK, wpk, 0);
   }
 }
 
@@ -1488,7 +1503,8 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
       return DMF_EDEVICE;
     }
   }
-  // Stepped grids: kGridBlocksPerCu blocks per CU (DMF_GRID_BLOCKS_PER_CU).
+  // Stepped grids: kGridBlocksPerCu blocks per CU (DMF_GRID_BLOCKS_PER_CU);
+  // push, repair and cascade launches kPushBlocksPerCu (DMF_PUSH_BLOCKS_PER_CU).
   // Cooperative grid: as many blocks as can be co-resident.
   int dev = 0, per_cu = 0;
   hipDeviceProp_t prop{};
@@ -1497,6 +1513,9 @@ int create_engine(const dmf_csr* g_caller, const std::vector<int32_t>& srcs_call
     const char* genv = std::getenv("DMF_GRID_BLOCKS_PER_CU");
     const int g = genv ? std::max(1, std::atoi(genv)) : kGridBlocksPerCu;
     E->grid_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E->num_cu)) * g));
+    const char* penv = std::getenv("DMF_PUSH_BLOCKS_PER_CU");
+    const int pg = penv ? std::max(1, std::atoi(penv)) : kPushBlocksPerCu;
+    E->push_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E->num_cu)) * pg));
     if (prop.cooperativeLaunch &&
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_solve_persistent),
                                                      kBlock, 0) == hipSuccess &&
@@ -2215,6 +2234,7 @@ int dmf_static_recalculate_noalloc(const dmf_csr* g, int32_t k, const int32_t* s
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     E.num_cu = prop.multiProcessorCount;
     E.grid_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E.num_cu)) * kGridBlocksPerCu));
+    E.push_cap = static_cast<int>(std::min<int64_t>(8192, static_cast<int64_t>(std::max(1, E.num_cu)) * kPushBlocksPerCu));
     Dev& D = E.D;
     D.n = n;
     D.m = m;

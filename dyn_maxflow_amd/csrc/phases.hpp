@@ -223,6 +223,63 @@ __device__ __forceinline__ bool is_src(const Dev& D, int v) { return D.term[v] &
 __device__ __forceinline__ bool is_snk(const Dev& D, int v) { return D.term[v] & 2; }
 __device__ __forceinline__ bool is_term(const Dev& D, int v) { return D.term[v] != 0; }
 
+// A pointer loaded from device memory (rather than taken from the kernel's
+// own arguments, which the compiler knows are global) is generic to the
+// compiler, and every access through it becomes a flat memory instruction:
+// flat loads count against the LDS/scalar wait counter as well, so each LDS or
+// scalar wait also drains every gather in flight.  as_global() rebuilds the
+// pointer from its address in the global address space; a plain cast there
+// and back would be folded away.
+template <class T>
+__device__ __forceinline__ T* as_global(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+// Every pointer of a Dev copied from device memory, as global pointers.
+__device__ __forceinline__ void globalize(Dev& D) {
+  D.term = as_global(D.term);
+  D.srcs = as_global(D.srcs);
+  D.snks = as_global(D.snks);
+  D.src_cbase = as_global(D.src_cbase);
+  D.snk_cbase = as_global(D.snk_cbase);
+  D.inst = as_global(D.inst);
+  D.iflow = as_global(D.iflow);
+  D.row = as_global(D.row);
+  D.col = as_global(D.col);
+  D.rev = as_global(D.rev);
+  D.cap = as_global(D.cap);
+  D.res = as_global(D.res);
+  D.rres = as_global(D.rres);
+  D.excess = as_global(D.excess);
+  D.height = as_global(D.height);
+  D.order = as_global(D.order);
+  D.wl = as_global(D.wl);
+  D.wl2 = as_global(D.wl2);
+  D.part = as_global(D.part);
+  D.side = as_global(D.side);
+  D.pfinal = as_global(D.pfinal);
+  D.ctrl = as_global(D.ctrl);
+  D.ctrs = as_global(D.ctrs);
+  D.lev16 = as_global(D.lev16);
+  D.prev = as_global(D.prev);
+  D.htab = as_global(D.htab);
+  D.owner = as_global(D.owner);
+  D.wq_vert = as_global(D.wq_vert);
+  D.wq_base = as_global(D.wq_base);
+  D.wide_vert = as_global(D.wide_vert);
+  D.wide_base = as_global(D.wide_base);
+  D.wave_vert = as_global(D.wave_vert);
+  D.wave_base = as_global(D.wave_base);
+  D.widebits = as_global(D.widebits);
+  D.bfs_ct = as_global(D.bfs_ct);
+  D.vmap = as_global(D.vmap);
+  D.tgt = as_global(D.tgt);
+  D.cmark = as_global(D.cmark);
+  D.usrc = as_global(D.usrc);
+  D.udst = as_global(D.udst);
+  D.uw = as_global(D.uw);
+  D.uarc = as_global(D.uarc);
+}
+
 __device__ __forceinline__ int ldr(const int32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -2327,7 +2384,8 @@ __global__ void __launch_bounds__(kBlock) k_bfs_level_dev(const Dev* __restrict_
     }
   }
   if (cnt == 0) return;  // uniform: the BFS is over
-  const Dev D = *pD;
+  Dev D = *pD;
+  globalize(D);
   // early-exit relabel whose targets are all labelled: this level is not
   // expanded, so the next one comes out empty.  A block that starts late may
   // see counts that blocks still running have raised and stop while others

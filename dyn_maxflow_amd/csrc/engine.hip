@@ -765,8 +765,12 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
       }
       if (topo) {
         const int fs = round & 1;
-        LAUNCH_PUSH(k_push, phase_blocks(n, static_chunks), D, 0, n, C.kc, PM_ALL, 0ull, fs);
-        LAUNCH_PUSH(k_repair, phase_blocks(n, static_chunks), D, 0, n, RM_ALL, 0ull);
+        // every vertex up to wave_lane one per lane, the static wave list for
+        // the rest
+        Dev Dt = D;
+        Dt.lane_max = D.wave_lane;
+        LAUNCH_PUSH(k_push, phase_blocks(n, static_chunks), Dt, 0, n, C.kc, PM_ALL, 0ull, fs);
+        LAUNCH_PUSH(k_repair, phase_blocks(n, static_chunks), Dt, 0, n, RM_ALL, 0ull);
         if ((rc = sync_ctrl(E))) return rc;
         if (!E->h_ctrl->flag[fs]) break;
         active = static_cast<double>(n);
@@ -927,6 +931,10 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
 
 int solve_persistent(dmf_engine* E, const Cfg& C) {
   Dev D = E->D;
+  // topology-driven: every vertex up to wave_lane one per lane, the static
+  // wave list for the rest (the persistent kernel's topology rounds use no
+  // queues)
+  if (C.variant == DMF_VARIANT_TOPO) D.lane_max = D.wave_lane;
   Cfg c = C;
   E->last_variant = C.variant;
   void* args[] = {&D, &c};
@@ -1109,12 +1117,16 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   std::vector<int32_t> wide_v, wide_b, wave_v, wave_b;
   int64_t wide_chunks = 0, wave_chunks = 0;
   // Vertices above lane_max: push and repair give each its own wave (one per
-  // chunk), from the active queues or, topology-driven, from the static wave
-  // list; the queues have a slot for every such vertex.
-  const int32_t wave_t = std::min(lane_max, wide_t);
+  // chunk) from the active queues, which have a slot for every such vertex.
+  // Topology-driven rounds take theirs from the static wave list of the
+  // vertices above wave_lane (the rest one per lane).
+  const int32_t queue_t = std::min(lane_max, wide_t);
+  const int32_t wave_lane = static_wave_t(lane_max, wide_t);
+  size_t nqueue_v = 0;
   for (int32_t v = 0; v < E->n; ++v) {
     const int32_t d = E->h_row[v + 1] - E->h_row[v];
-    if (d > wave_t) {
+    if (d > queue_t) ++nqueue_v;
+    if (d > wave_lane) {
       wave_v.push_back(v);
       wave_b.push_back(static_cast<int32_t>(wave_chunks));
       wave_chunks += (d + kChunk - 1) / kChunk;
@@ -1126,7 +1138,8 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
     }
   }
   const size_t nwide = std::max<size_t>(1, wide_v.size());
-  const size_t nqueue = std::max<size_t>(1, wave_v.size());
+  const size_t nqueue = std::max<size_t>(1, nqueue_v);
+  const size_t nwave = std::max<size_t>(1, wave_v.size());
   std::vector<uint32_t> wbits((static_cast<size_t>(E->n) + 31) / 32 + 1, 0u);
   for (int32_t v : wide_v) wbits[static_cast<uint32_t>(v) >> 5] |= 1u << (v & 31);
   HIP_OK(hipStreamSynchronize(E->stream));  // no kernel uses the old lists from here on
@@ -1142,7 +1155,7 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   int rc;
   if (!(rc = dalloc(E, &wq_vert, nqueue * kNumQ)) && !(rc = dalloc(E, &wq_base, nqueue * kNumQ)) &&
       !(rc = dalloc(E, &wide_vert, nwide)) && !(rc = dalloc(E, &wide_base, nwide)) &&
-      !(rc = dalloc(E, &wave_vert, nqueue)) && !(rc = dalloc(E, &wave_base, nqueue)) &&
+      !(rc = dalloc(E, &wave_vert, nwave)) && !(rc = dalloc(E, &wave_base, nwave)) &&
       !(rc = dalloc(E, &widebits, wbits.size())) && !(rc = dalloc(E, &bfs_ct, ct_cap * kNumRing))) {
     const bool copied = put(wide_vert, wide_v.data(), sizeof(int32_t) * wide_v.size()) &&
                         put(wide_base, wide_b.data(), sizeof(int32_t) * wide_b.size()) &&
@@ -1172,6 +1185,7 @@ int setup_wide(dmf_engine* E, int32_t lane_max, int32_t wide_t) {
   // BFS segments captured for the old lists launch the old chunk count's grid
   destroy_segments(E);
   D.lane_max = lane_max;
+  D.wave_lane = wave_lane;
   D.wq_vert = wq_vert;
   D.wq_base = wq_base;
   D.wq_cap = static_cast<int32_t>(nqueue);
@@ -2010,7 +2024,7 @@ int64_t dmf_engine_footprint(int32_t n, int32_t m) {
   b += 4 * (N / 32 + 1);                        // wide-vertex bitmap
   b += 8 * (M / 64 + 1);                        // static wide list (wide_t >= 64)
   b += 3 * 16 * (M / kChunk + M / 64 + 1);      // BFS ring chunk tables: chunks of the vertices above wide_t
-  b += 48 * std::min(N + 1, M / 17 + 1);        // queues and wave list: per vertex above lane_max (>= 16)
+  b += 48 * std::min(N + 1, M / 5 + 1);         // queues and wave list: per vertex above lane_max (>= 4)
   b += static_cast<int64_t>(C_N) * kSlots * 8;  // counters
   b += int64_t{1} << 20;                        // control block, scheduler, allocation slack
   return b;
@@ -2078,7 +2092,7 @@ int dmf_batch_apply(dmf_engine* E, const int32_t* counts, const int32_t* src, co
 int dmf_engine_autotune(dmf_engine* E, const dmf_params* p, int32_t* lane_max, int32_t* wide_t) {
   return guarded([&]() -> int {
     if (!E) return DMF_EINVAL;
-    static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {16, 64}};
+    static const int32_t cand[][2] = {{16, 128}, {32, 256}, {32, 1024}, {64, 512}, {16, 64}, {4, 64}, {4, 128}};
     double best_ms = 1e300;
     int best = 1;
     for (int c = 0; c < static_cast<int>(sizeof(cand) / sizeof(cand[0])); ++c) {
@@ -2244,6 +2258,7 @@ int dmf_static_recalculate_noalloc(const dmf_csr* g, int32_t k, const int32_t* s
     D.ninst = 1;
     D.lane_max = kLaneMax;
     D.wide_t = kWideT;
+    D.wave_lane = static_wave_t(kLaneMax, kWideT);
     D.row = na_at<int32_t>(base, L.row);
     D.col = na_at<int32_t>(base, L.col);
     D.rev = na_at<int32_t>(base, L.rev);

@@ -35,7 +35,19 @@ namespace dmfdev {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kLaneMax = 16;   // default of Dev::lane_max (thresholds are tunable per engine)
+// Default of Dev::lane_max (thresholds are tunable per engine).  A vertex of
+// degree 5-16 discharged by one lane serially held its wave for the longest
+// scan; on RMAT-20 batches lane_max 4 measured learned p50 2.87 vs 3.04 ms at
+// 16 and plain 4.45 vs 4.58 ms (2-8 all within 1% of 4; 32 slower).  4 keeps
+// grid graphs (degree <= 4) on the lane path.
+constexpr int kLaneMax = 4;
+// Topology-driven push and repair visit every vertex each round: there the
+// lane path keeps degrees up to 16 (wave_lane), which measured faster than 4.
+constexpr int kTopoLaneMax = 16;
+__host__ __device__ inline int32_t static_wave_t(int32_t lane_max, int32_t wide_t) {
+  const int32_t a = lane_max > kTopoLaneMax ? lane_max : kTopoLaneMax;
+  return a < wide_t ? a : wide_t;
+}
 constexpr int kWideT = 64;     // default of Dev::wide_t
 constexpr int kChunk = 256;
 constexpr int kGroup = 16;      // frontier vertices per wave task in the flattened BFS expansion
@@ -132,6 +144,8 @@ struct Dev {
   int32_t ninst;
   int32_t lane_max;     // degree up to which one lane scans a vertex
   int32_t wide_t;       // degree above which a vertex is split into chunks
+  int32_t wave_lane;    // the static wave list holds the vertices above this degree
+                        // (topology-driven rounds take the rest one per lane)
   const int32_t* row;
   const int32_t* col;
   const int32_t* rev;
@@ -2639,15 +2653,15 @@ __global__ void __launch_bounds__(kBlock) k_build_imbalance_wl(Dev D, int cnt, i
 __global__ void __launch_bounds__(kBlock) k_count_imbalance(Dev D) { ph_count_imbalance(D, make_range()); }
 
 // Static lists of the vertices above the degree thresholds (the wide list of
-// degree > wide_t with its bitmap, and the wave list of degree > min(lane_max,
-// wide_t)), built on the device for engines that have no host copy of the CSR.
+// degree > wide_t with its bitmap, and the wave list of degree > wave_lane),
+// built on the device for engines that have no host copy of the CSR.
 // packed[0] / packed[1] receive the wide / wave list words (slots << 32 |
 // chunks); both and widebits must be zero on entry.
 __global__ void __launch_bounds__(kBlock) k_build_static_lists(Dev D, int32_t* wide_vert, int32_t* wide_base,
                                                                int32_t* wave_vert, int32_t* wave_base,
                                                                uint32_t* widebits, unsigned long long* packed) {
   const Range r = make_range();
-  const int wave_t = min(D.lane_max, D.wide_t);
+  const int wave_t = D.wave_lane;
   for (int base = r.gid - lane_id(); base < D.n; base += r.gsz) {
     const int v = base + lane_id();
     const int d = v < D.n ? degree(D, v) : 0;

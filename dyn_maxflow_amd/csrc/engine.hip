@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -135,6 +136,8 @@ struct dmf_engine {
   bool use_graph = false;
   bool graph_failed = false;
   int seg_levels[kBfsModes] = {};
+  // the same for early-exit relabels from t, which stop shallower than full ones
+  int seg_levels_early = 0;
   // round graphs (mode 3): a whole data-driven round with 2*i level launches
   // in slot i, captured for push/relabel cycles kc (a kernel argument fixed
   // at capture); round_depth is the deepest relabel of the last such solve
@@ -144,6 +147,18 @@ struct dmf_engine {
   int round_kc[kRoundSlots] = {};
   int round_depth = 0;
   bool use_rounds = false;
+  // A round queued behind a relabel's first level segment (data-driven loops
+  // with cascade sub-rounds): round_behind launches it; relabel_graph records
+  // from its first host check whether it ran (spec_ran: the BFS was over and
+  // the list not empty), the list sizes it used and how much of the cascade
+  // chain it queued.
+  const std::function<int()>* round_behind = nullptr;
+  bool spec_launched = false;
+  bool spec_ran = false;
+  int32_t spec_cnt = 0;
+  unsigned long long spec_wpk = 0;
+  int spec_sub = 0;
+  SchedRound spec_sr{};
   // per-launch BFS level timing (dmf_engine_set_profiling)
   bool profile = false;
   hipEvent_t pev0 = nullptr, pev1 = nullptr;
@@ -403,13 +418,16 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
     return DMF_OK;
   };
   bool first = true;
+  E->spec_launched = E->spec_ran = false;
   hipLaunchKernelGGL(k_relabel_prep, dim3(1), dim3(kWave), 0, E->stream, D, E->d_dev, early ? 1 : 0);
   ++E->launches;
   HIP_OK(hipGetLastError());
   if (mode == BFS_FINAL) LAUNCH(k_bfs_init_final, blocks_for(part_cnt), D, part_cnt);
   else LAUNCH(k_bfs_init, blocks_for(E->n), D, mode, early ? 1 : 0);
   int idx = 1;
-  while (idx + 1 < dmf_engine::kSegSizes && (1 << idx) < E->seg_levels[mode] + 1) ++idx;
+  // the first segment as long as the last relabel of the same kind was deep
+  const int predicted = early ? E->seg_levels_early : E->seg_levels[mode];
+  while (idx + 1 < dmf_engine::kSegSizes && (1 << idx) < predicted + 1) ++idx;
   int lvl = 0;
   for (;;) {
     if ((rc = capture_segment(E, mode, idx))) return rc;
@@ -418,8 +436,18 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
     if (E->profile) HIP_OK(hipEventRecord(E->pev1, E->stream));
     ++E->launches;
     if (list && first && (rc = list_behind())) return rc;
+    const bool spec = list && first && E->round_behind != nullptr;
+    if (spec && (rc = (*E->round_behind)())) return rc;
+    const bool first_check = first;
     first = false;
     if ((rc = sync_ctrl(E))) return rc;
+    if (first_check && spec) {
+      const Ctrl& h = *E->h_ctrl;
+      E->spec_launched = true;
+      E->spec_ran = !h.bfs_inc && !h.r_skip;
+      E->spec_cnt = h.r_cnt;
+      E->spec_wpk = h.r_wpk;
+    }
     if (E->profile) {
       float ms = 0.f;
       HIP_OK(hipEventElapsedTime(&ms, E->pev0, E->pev1));
@@ -427,7 +455,9 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
       E->last.bfs_level_launches += static_cast<uint64_t>(1) << idx;
     }
     lvl = E->h_ctrl->bcur[0][1];
-    if (E->h_ctrl->qcnt[lvl % 3] == 0) break;
+    // behind the first segment, k_relabel_done recorded whether the BFS was
+    // over (a round queued there reuses the ring counters once it was)
+    if (first_check && list ? !E->h_ctrl->bfs_inc : E->h_ctrl->qcnt[lvl % 3] == 0) break;
     if (lvl > E->n + 1) return DMF_EDEVICE;
     if (idx + 1 < dmf_engine::kSegSizes) ++idx;
   }
@@ -436,7 +466,7 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
     if ((rc = list_behind())) return rc;
     if ((rc = sync_ctrl(E))) return rc;
   }
-  E->seg_levels[mode] = lvl;
+  (early ? E->seg_levels_early : E->seg_levels[mode]) = lvl;
   E->last.bfs_levels += lvl;
   E->last.global_relabels += 1;
   E->last_levels = lvl;
@@ -720,9 +750,49 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         return rc;
       }
     }
+    const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
+    // the push grid: the first sub-rounds carry lists as long as the round's
+    // own; 2 blocks per CU measured 2-3% slower per learned batch on RMAT-20
+    // than 4
+    const int cgrid = std::max(1, E->push_cap);
+    // the scheduler's round-end decision rides on the first cascade launch
+    // that finds the chain's list empty
+    const SchedWeights* sw = consult ? E->d_sched_w : nullptr;
+    // With cascade sub-rounds, a round that follows a graph-replayed relabel is
+    // queued behind the relabel's first level segment and worklist: push and
+    // repair size themselves from the list k_list_snap records on the device,
+    // and the chain's first kCascadeChunk sub-rounds (with the scheduler's
+    // decision) follow, so one host check serves the relabel, the list and
+    // the round.  When the BFS outlasts that segment, or the list is empty,
+    // the queued kernels do nothing and the round proceeds as below.
+    const std::function<int()> round_behind = [&]() -> int {
+      int rc2;
+      if (E->cstamp > INT_MAX - (sub + 2)) {
+        if ((rc2 = zero_dev(E, D.cmark, sizeof(int32_t) * static_cast<size_t>(n)))) return rc2;
+        E->cstamp = 0;
+      }
+      hipLaunchKernelGGL(k_list_snap, dim3(1), dim3(kWave), 0, E->stream, D);
+      ++E->launches;
+      HIP_OK(hipGetLastError());
+      LAUNCH_PUSH(k_push, E->push_cap, D, 1, -1, C.kc, PM_ALL, 0ull, 0, 0, ++E->cstamp, E->cascade_relist);
+      LAUNCH_PUSH(k_repair, E->push_cap, D, 1, -1, RM_ALL, 0ull);
+      // rounds since the relabel: 1; active and the relabel's depth: on the device
+      E->spec_sr = SchedRound{D.k, inc_fraction(E), 1, -1, -1.0, ++E->sched_tag, 1};
+      const int chunk = std::min(kCascadeChunk, sub);
+      int slot = 0;
+      for (int j = 0; j < chunk; ++j) {
+        LAUNCH_PUSH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist, sw, E->d_sched, E->spec_sr);
+        slot = (slot + 1) % 3;
+      }
+      E->spec_sub = chunk;
+      if (consult)
+        HIP_OK(hipMemcpyAsync(E->h_sched, E->d_sched, sizeof(SchedState), hipMemcpyDeviceToHost, E->stream));
+      return DMF_OK;
+    };
     for (; !rounds;) {
       bool fresh = false;
       bool listed = false;  // the worklist was built behind this round's relabel
+      bool spec_ran = false;  // ... and the round itself, with its first cascade chunk
       const bool due = learned ? (want_relabel || since >= 16) : since >= C.interval;
       if (due) {
         // the relabels that may stop early may also restrict their bottom-up
@@ -730,7 +800,11 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         const bool early = try_early && !need_full;
         // data-driven rounds build their worklist behind the relabel (one
         // host check for both) where the levels are graph-replayed
-        if ((rc = relabel(E, BFS_SINK, 0, &visited, early, early, topo ? nullptr : &listed))) return rc;
+        E->round_behind = (!topo && sub > 0 && round < C.max_rounds) ? &round_behind : nullptr;
+        rc = relabel(E, BFS_SINK, 0, &visited, early, early, topo ? nullptr : &listed);
+        E->round_behind = nullptr;
+        if (rc) return rc;
+        spec_ran = listed && E->spec_launched && E->spec_ran;
         since = 0;
         const bool cut = early && E->h_ctrl->bfs_cut;
         fresh = !cut && !E->restricted;
@@ -749,8 +823,10 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
           LAUNCH(k_build_wl, blocks_for(all ? n : visited), D, all ? -1 : visited);
           if ((rc = sync_ctrl(E))) return rc;
         }
-        cnt = E->h_ctrl->wl_cnt;
-        wpk = E->h_ctrl->wq[kQActive];
+        // a round queued behind the relabel took the list sizes on the
+        // device and cleared the counters
+        cnt = spec_ran ? E->spec_cnt : E->h_ctrl->wl_cnt;
+        wpk = spec_ran ? E->spec_wpk : E->h_ctrl->wq[kQActive];
         if (cnt == 0 && wpk == 0) {
           if (fresh) break;
           since = C.interval;  // confirm convergence on fresh labels
@@ -775,30 +851,38 @@ int solve_stepped(dmf_engine* E, const Cfg& C0) {
         if (!E->h_ctrl->flag[fs]) break;
         active = static_cast<double>(n);
       } else {
-        if (sub > 0 && E->cstamp > INT_MAX - (sub + 2)) {
-          // list stamps only grow: start them again from a cleared mark array
-          if ((rc = zero_dev(E, D.cmark, sizeof(int32_t) * static_cast<size_t>(n)))) return rc;
-          E->cstamp = 0;
-        }
-        LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
-               sub ? ++E->cstamp : 0, E->cascade_relist);
-        LAUNCH_PUSH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
         active = static_cast<double>(cnt) + static_cast<double>(wpk >> 32);
+        int slot = 0, done_sub = 0;
+        SchedRound sr{};
+        bool chain_over = sub == 0;
+        if (spec_ran) {
+          // push, repair and the chain's first sub-rounds ran behind the
+          // relabel; its host check also read the chain's state
+          sr = E->spec_sr;
+          done_sub = E->spec_sub;
+          slot = done_sub % 3;
+          if (done_sub >= sub) {
+            chain_over = true;
+          } else if (E->h_ctrl->qcnt[slot] == 0 && E->h_ctrl->wq[slot] == 0) {
+            sched_done = consult && E->h_sched->tag == sr.tag;
+            chain_over = true;
+          }
+        } else {
+          if (sub > 0 && E->cstamp > INT_MAX - (sub + 2)) {
+            // list stamps only grow: start them again from a cleared mark array
+            if ((rc = zero_dev(E, D.cmark, sizeof(int32_t) * static_cast<size_t>(n)))) return rc;
+            E->cstamp = 0;
+          }
+          LAUNCH_PUSH(k_push, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, C.kc, PM_ALL, wpk, 0, sub ? 0 : -1,
+                      sub ? ++E->cstamp : 0, E->cascade_relist);
+          LAUNCH_PUSH(k_repair, phase_blocks(cnt, chunks_of(wpk)), D, 1, cnt, RM_ALL, wpk);
+          sr = SchedRound{D.k, inc_fraction(E), since + 1, E->last_levels, active, ++E->sched_tag, 0};
+        }
         // cascade sub-rounds: each pushes from the list the pass before it
         // wrote, launched kCascadeChunk at a time with one host check between
         // chunks (an empty list ends the chain)
-        const int ckc = E->cascade_kc > 0 ? E->cascade_kc : C.kc;
-        if (sub > 0) {
-          // the push grid: the first sub-rounds carry lists as long as the
-          // round's own; 2 blocks per CU measured 2-3% slower per learned
-          // batch on RMAT-20 than 4
-          const int cgrid = std::max(1, E->push_cap);
-          // the scheduler's round-end decision rides on the first launch
-          // that finds the chain's list empty
-          const SchedWeights* sw = consult ? E->d_sched_w : nullptr;
-          const SchedRound sr{D.k, inc_fraction(E), since + 1, E->last_levels, active, ++E->sched_tag};
-          int slot = 0;
-          for (int done_sub = 0; done_sub < sub;) {
+        if (!chain_over) {
+          for (; done_sub < sub;) {
             const int chunk = std::min(kCascadeChunk, sub - done_sub);
             for (int j = 0; j < chunk; ++j) {
               LAUNCH_PUSH(k_cascade, cgrid, D, slot, ckc, ++E->cstamp, E->cascade_relist, sw, E->d_sched, sr);

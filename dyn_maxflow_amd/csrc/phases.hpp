@@ -2295,6 +2295,26 @@ __global__ void __launch_bounds__(kBlock) k_build_wl_after(Dev D) {
   ph_build_wl(D, make_range(), cnt > D.n / 4 ? -1 : cnt, t);
   t.flush(D.ctrs);
 }
+// k_list_snap: the round's list sizes, for push, repair and cascade launches
+// queued behind a relabel before the host has read them.  The round is
+// skipped (r_skip) when the BFS is not over (bfs_inc: the list was not built)
+// or the list is empty; otherwise the list counters are cleared for the next
+// round's builder, as a host-sized push launch does.  One wave.
+__global__ void k_list_snap(Dev D) {
+  if (threadIdx.x != 0) return;
+  Ctrl* c = D.ctrl;
+  const int inc = c->bfs_inc;
+  const int cnt = c->wl_cnt;
+  const unsigned long long wpk = c->wq[kQActive];
+  const bool skip = inc || (cnt == 0 && wpk == 0);
+  c->r_skip = skip ? 1 : 0;
+  c->r_cnt = inc ? 0 : cnt;
+  c->r_wpk = inc ? 0ull : wpk;
+  if (!inc) {
+    c->wl_cnt = c->wl2_cnt = 0;
+    c->wq[kQActive] = c->wq[kQPull] = 0;
+  }
+}
 // k_flow_prep: list counters, flags, the flow and the per-instance flows,
 // before the flow reduction.
 __global__ void __launch_bounds__(kBlock) k_flow_prep(Dev D) {
@@ -2448,9 +2468,18 @@ __global__ void __launch_bounds__(kBlock) k_build_final_wl(Dev D, int cnt) {
 // flag[fslot] and clears flag[fslot ^ 1], which the host read a round ago.
 // cslot >= 0 (which == 1 only): also list, in cascade slot cslot with stamp
 // cstamp, the vertices the first cascade sub-round takes.
+// cnt < 0 (which == 1): the list sizes come from the snapshot k_list_snap
+// took (r_cnt, r_wpk), and the launch does nothing when it marked the round
+// skipped (r_skip).
 __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int kc, int pmode,
                                                  unsigned long long wpk, int fslot, int cslot = -1,
                                                  int cstamp = 0, int crelist = 0) {
+  if (cnt < 0) {
+    const Ctrl* cc = D.ctrl;
+    if (cc->r_skip) return;  // uniform
+    cnt = cc->r_cnt;
+    wpk = cc->r_wpk;
+  }
   Tally t;
   const Range r = make_range();
   if (is_block0_thread0()) {
@@ -2473,6 +2502,12 @@ __global__ void __launch_bounds__(kBlock) k_push(Dev D, int which, int cnt, int 
 }
 __global__ void __launch_bounds__(kBlock) k_repair(Dev D, int which, int cnt, int rmode,
                                                    unsigned long long wpk) {
+  if (cnt < 0) {  // as k_push
+    const Ctrl* cc = D.ctrl;
+    if (cc->r_skip) return;
+    cnt = cc->r_cnt;
+    wpk = cc->r_wpk;
+  }
   Tally t;
   const Range r = make_range();
   if (which == 0) ph_repair(D, r, nullptr, cnt, rmode, wlist_waves(D), t);

@@ -152,6 +152,10 @@ struct SchedRound {
   int last_levels;
   double active;
   int32_t tag;  // the round's tag: the decision is taken once per tag
+  // the chain was queued behind the relabel, before the host read the round's
+  // list: a round k_list_snap skipped takes no decision, and active and
+  // last_levels come from the control block (r_cnt, r_wpk; the cursor's level)
+  int32_t spec;
 };
 
 // One cascade sub-round: pushes from the list in slot `slot` (sizes read from
@@ -167,6 +171,9 @@ struct SchedRound {
 // it with the round's tag.
 __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int stamp, int relist,
                                                     const SchedWeights* W, SchedState* S, SchedRound a) {
+  // a chain queued behind a relabel whose round k_list_snap skipped does
+  // nothing: while the BFS goes on, the ring slots are the BFS's
+  if (a.spec && D.ctrl->r_skip) return;  // uniform
   // block 0 loads the decision tag beside the list sizes (one round trip for
   // the empty launches that end every chain)
   const bool decider = W != nullptr && blockIdx.x == 0;
@@ -187,7 +194,10 @@ __global__ void __launch_bounds__(kBlock) k_cascade(Dev D, int slot, int kc, int
       for (int i = threadIdx.x; i < kSchedParams; i += blockDim.x) dst[i] = src[i];
       __syncthreads();
       if (threadIdx.x < kWave) {
-        sched_eval<false>(D, &wl, S, 1, a.k, a.inc_frac, a.rounds_since, a.last_levels, a.active);
+        const double active = a.spec ? static_cast<double>(cc->r_cnt) + static_cast<double>(cc->r_wpk >> 32)
+                                     : a.active;
+        const int last_levels = a.spec ? cc->bcur[0][1] : a.last_levels;
+        sched_eval<false>(D, &wl, S, 1, a.k, a.inc_frac, a.rounds_since, last_levels, active);
         if (threadIdx.x == 0) str(&S->tag, a.tag);
       }
     }

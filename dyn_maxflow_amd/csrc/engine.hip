@@ -399,21 +399,20 @@ int capture_round(dmf_engine* E, int idx, int kc) {
 
 // Global relabel with the level loop replayed from captured graphs: one host
 // check per segment instead of one per level.  The first segment is as long as
-// the previous BFS of this mode was deep; later ones grow.  Same result as
-// relabel_stepped (the same level kernel body runs on the same frontiers).
+// the previous BFS of the same kind was deep; later ones grow.  Same result
+// as relabel_stepped (the same level kernel body runs on the same frontiers).
 // list (BFS_SINK): the worklist of the round that follows is built behind the
-// first segment (k_relabel_done, k_cap_after for an early-exit relabel,
-// k_build_wl_after) and taken from the same host check; a deeper BFS runs on
-// and the three are launched again once it is over.
+// first segment (k_cap_after for an early-exit relabel, k_build_wl_after) and
+// taken from the same host check; a deeper BFS runs on and the two are
+// launched again once it is over.
 int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool early, bool list) {
   Dev& D = E->D;
   int rc;
   const int gn = std::min(blocks_for(E->n), E->grid_cap);
   auto list_behind = [&]() -> int {
-    hipLaunchKernelGGL(k_relabel_done, dim3(1), dim3(kWave), 0, E->stream, D);
     if (early) hipLaunchKernelGGL(k_cap_after, dim3(gn), dim3(kBlock), 0, E->stream, D);
     hipLaunchKernelGGL(k_build_wl_after, dim3(gn), dim3(kBlock), 0, E->stream, D);
-    E->launches += early ? 3 : 2;
+    E->launches += early ? 2 : 1;
     HIP_OK(hipGetLastError());
     return DMF_OK;
   };
@@ -455,8 +454,8 @@ int relabel_graph(dmf_engine* E, int mode, int part_cnt, int* visited, bool earl
       E->last.bfs_level_launches += static_cast<uint64_t>(1) << idx;
     }
     lvl = E->h_ctrl->bcur[0][1];
-    // behind the first segment, k_relabel_done recorded whether the BFS was
-    // over (a round queued there reuses the ring counters once it was)
+    // behind the first segment, k_build_wl_after recorded whether the BFS
+    // was over (a round queued there reuses the ring counters once it was)
     if (first_check && list ? !E->h_ctrl->bfs_inc : E->h_ctrl->qcnt[lvl % 3] == 0) break;
     if (lvl > E->n + 1) return DMF_EDEVICE;
     if (idx + 1 < dmf_engine::kSegSizes) ++idx;

@@ -95,7 +95,7 @@ struct Ctrl {
   int32_t early;      // the current relabel may stop once its targets are labelled
   int32_t bfs_cut;    // the current relabel stopped before its frontier ran out
   int32_t bfs_rbu;    // the current relabel ran a restricted bottom-up level
-  int32_t bfs_inc;    // k_relabel_done: the relabel's frontier was not empty yet
+  int32_t bfs_inc;    // k_build_wl_after: the relabel's frontier was not empty yet
   int32_t bcur[2][2]; // graph-replayed BFS cursors {first order[] index, level},
                       // read from slot r and written to slot r^1 by each level
   // Round-graph driver: loop state of the data-driven fixed schedule kept on
@@ -2256,41 +2256,45 @@ __global__ void __launch_bounds__(kBlock) k_cap_unlabelled(Dev D, int cap) {
     if (D.height[v] == D.n && D.excess[v] <= 0) D.height[v] = cap;
 }
 // Worklist behind a relabel's first replayed segment, without a host check in
-// between: k_relabel_done records whether the BFS is over (the frontier at the
-// cursor, slot 0 at a segment's end, is empty) and its labelled count;
-// k_cap_after and k_build_wl_after do nothing while it is not, and the host,
-// seeing bfs_inc, runs further segments and launches the three again.
-__global__ void k_relabel_done(Dev D) {
-  if (threadIdx.x != 0) return;
-  Ctrl* c = D.ctrl;
-  // the ring counts are loaded beside the cursor (one round trip)
+// between: each kernel finds from the level cursor (slot 0 at a segment's end)
+// whether the BFS is over, i.e. the frontier there is empty, and does nothing
+// while it is not; k_build_wl_after records the answer (bfs_inc) and the
+// labelled count, and the host, seeing bfs_inc, runs further segments and
+// launches the two again.  The ring counts are loaded beside the cursor (one
+// round trip).  k_build_wl_after's block 0 clears the ring once the BFS is
+// over, which leaves the count at the cursor zero, so every block agrees.
+__device__ __forceinline__ bool bfs_over(const Ctrl* c, int* labelled) {
   const int2 cur = *reinterpret_cast<const int2*>(&c->bcur[0][0]);
   const int4 qc = *reinterpret_cast<const int4*>(&c->qcnt[0]);
   const int slot = cur.y % 3;
-  const int inc = (slot == 0 ? qc.x : (slot == 1 ? qc.y : qc.z)) != 0 ? 1 : 0;
-  c->bfs_inc = inc;
-  if (!inc) c->visited = cur.x;
+  *labelled = cur.x;
+  return (slot == 0 ? qc.x : (slot == 1 ? qc.y : qc.z)) == 0;
 }
 // k_cap_unlabelled after an early-exit relabel that stopped short (bfs_cut),
 // with the cap taken from the level cursor.
 __global__ void __launch_bounds__(kBlock) k_cap_after(Dev D) {
   const Ctrl* c = D.ctrl;
-  if (c->bfs_inc || !c->bfs_cut) return;
+  int labelled;
+  if (!bfs_over(c, &labelled) || !c->bfs_cut) return;
   const int cap = max(0, c->bcur[0][1] - 1);
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < D.n; v += gridDim.x * blockDim.x)
     if (D.height[v] == D.n && D.excess[v] <= 0) D.height[v] = cap;
 }
-// k_build_wl over the labelled count k_relabel_done recorded (every vertex
-// when that is a quarter of the graph or more).
+// k_build_wl over the labelled count (every vertex when that is a quarter of
+// the graph or more).
 __global__ void __launch_bounds__(kBlock) k_build_wl_after(Dev D) {
-  const Ctrl* cc = D.ctrl;
-  if (cc->bfs_inc) return;  // block-uniform
-  const int cnt = cc->visited;
+  int cnt;
+  const bool over = bfs_over(D.ctrl, &cnt);  // block-uniform
   if (is_block0_thread0()) {
     Ctrl* c = D.ctrl;
-    c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
-    c->wq[0] = c->wq[1] = c->wq[2] = 0;
+    c->bfs_inc = over ? 0 : 1;
+    if (over) {
+      c->visited = cnt;
+      c->qcnt[0] = c->qcnt[1] = c->qcnt[2] = 0;
+      c->wq[0] = c->wq[1] = c->wq[2] = 0;
+    }
   }
+  if (!over) return;
   Tally t;
   ph_build_wl(D, make_range(), cnt > D.n / 4 ? -1 : cnt, t);
   t.flush(D.ctrs);

@@ -366,7 +366,7 @@ int capture_round(dmf_engine* E, int idx, int kc) {
   const Dev& D = E->D;
   const int gn = std::min(blocks_for(E->n), E->grid_cap);
   const int glev = std::min(phase_blocks(E->n, chunks_of(D.wide_packed)), E->grid_cap);
-  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->grid_cap);
+  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->push_cap);  // push and repair
   hipStream_t st = E->stream;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) return kNoGraph;
   hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(kWave), 0, st, D);
@@ -599,7 +599,7 @@ int rounds_loop(dmf_engine* E, const Cfg& C, int* round, int* visited, bool* con
   // rounds as long as the deepest relabel of the last solve of this driver
   int idx = round_slot(E->round_depth);
   const int gn = std::min(blocks_for(E->n), E->grid_cap);
-  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->grid_cap);
+  const int glist = std::min(phase_blocks(E->n, chunks_of(D.wave_packed)), E->push_cap);  // push and repair
   int deepest = 0;
   for (;;) {
     if ((rc = capture_round(E, idx, C.kc))) return rc;

@@ -61,6 +61,8 @@ struct dmf_engine {
   int32_t* d_udst = nullptr;
   int32_t* d_uw = nullptr;
   int32_t* d_uarc = nullptr;
+  int32_t* d_urev = nullptr;
+  int32_t* d_udelta = nullptr;
   int32_t ucap = 0;
   int coop_blocks = 0;
   int num_cu = 0;
@@ -1168,6 +1170,10 @@ int complete_phase(dmf_engine* E, bool deficits, int kc, int max_rounds) {
   }
 }
 
+// Device bytes per update record an engine holds (source, destination, new
+// capacity, arc, reverse arc, capacity change).
+constexpr int64_t kUpdateSlotBytes = 24;
+
 int ensure_update_capacity(dmf_engine* E, int32_t k) {
   if (k <= E->ucap) return DMF_OK;
   int32_t cap = std::max<int32_t>(k, 1024);
@@ -1175,12 +1181,15 @@ int ensure_update_capacity(dmf_engine* E, int32_t k) {
   hipFree(E->d_udst);
   hipFree(E->d_uw);
   hipFree(E->d_uarc);
-  E->bytes -= static_cast<int64_t>(E->ucap) * 16;
-  E->d_usrc = E->d_udst = E->d_uw = E->d_uarc = nullptr;
+  hipFree(E->d_urev);
+  hipFree(E->d_udelta);
+  E->bytes -= static_cast<int64_t>(E->ucap) * kUpdateSlotBytes;
+  E->d_usrc = E->d_udst = E->d_uw = E->d_uarc = E->d_urev = E->d_udelta = nullptr;
   E->ucap = 0;
   int rc;
   if ((rc = dalloc(E, &E->d_usrc, cap)) || (rc = dalloc(E, &E->d_udst, cap)) ||
-      (rc = dalloc(E, &E->d_uw, cap)) || (rc = dalloc(E, &E->d_uarc, cap)))
+      (rc = dalloc(E, &E->d_uw, cap)) || (rc = dalloc(E, &E->d_uarc, cap)) ||
+      (rc = dalloc(E, &E->d_urev, cap)) || (rc = dalloc(E, &E->d_udelta, cap)))
     return rc;
   E->ucap = cap;
   return DMF_OK;
@@ -1882,6 +1891,8 @@ int dmf_engine_destroy(dmf_engine* E) {
     hipFree(E->d_udst);
     hipFree(E->d_uw);
     hipFree(E->d_uarc);
+    hipFree(E->d_urev);
+    hipFree(E->d_udelta);
     hipFree(E->d_vmap);
     destroy_segments(E);
     hipFree(E->d_dev);
@@ -1954,6 +1965,8 @@ int dmf_engine_apply_batch_device(dmf_engine* E, int32_t k, const int32_t* d_src
     E->D.udst = d_dst;
     E->D.uw = d_w;
     E->D.uarc = E->d_uarc;
+    E->D.urev = E->d_urev;
+    E->D.udelta = E->d_udelta;
     E->D.k = k;
     rc = run_solve(E, p, 1, flow);
     E->D.k = 0;
@@ -2272,6 +2285,8 @@ int dmf_static_recalculate(const dmf_csr* g, int32_t k, const int32_t* src, cons
         D.udst = E->d_udst;
         D.uw = E->d_uw;
         D.uarc = E->d_uarc;
+        D.urev = E->d_urev;
+        D.udelta = E->d_udelta;
         D.k = k;
         hipLaunchKernelGGL(k_upd_lookup, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D);
         hipLaunchKernelGGL(k_upd_apply, dim3(blocks_for(k)), dim3(kBlock), 0, E->stream, D, 1, 0);

@@ -213,6 +213,11 @@ struct Dev {
   const int32_t* udst;
   const int32_t* uw;
   int32_t* uarc;
+  // per record, written by the lookup where non-null: the reverse arc of the
+  // record's arc and its capacity change (the apply and the clamp then read
+  // them as streams instead of gathering rev[] and cap[] again)
+  int32_t* urev;
+  int32_t* udelta;
   int32_t k;
 };
 
@@ -292,6 +297,8 @@ __device__ __forceinline__ void globalize(Dev& D) {
   D.udst = as_global(D.udst);
   D.uw = as_global(D.uw);
   D.uarc = as_global(D.uarc);
+  D.urev = as_global(D.urev);
+  D.udelta = as_global(D.udelta);
 }
 
 __device__ __forceinline__ int ldr(const int32_t* p) {
@@ -2138,7 +2145,12 @@ __device__ void ph_upd_lookup(const Dev& D, const Range& r) {
         ++missing;
       } else {
         atomicMax(&D.owner[arc], i);
-        inc += D.uw[i] > D.cap[arc];
+        const int c0 = D.cap[arc];  // the apply changes it only after this kernel
+        inc += D.uw[i] > c0;
+        if (D.urev) {
+          D.urev[i] = D.rev[arc];
+          D.udelta[i] = D.uw[i] - c0;
+        }
       }
     }
     D.uarc[i] = arc;
@@ -2168,9 +2180,9 @@ __device__ void ph_upd_apply(const Dev& D, const Range& r, bool capacity_only, b
     }
     const int nw = D.uw[i];
     if (!capacity_only) {
-      const int delta = nw - D.cap[arc];
+      const int delta = D.udelta ? D.udelta[i] : nw - D.cap[arc];
       D.res[arc] += delta;
-      D.rres[D.rev[arc]] += delta;
+      D.rres[D.urev ? D.urev[i] : D.rev[arc]] += delta;
     }
     D.cap[arc] = nw;
     D.owner[arc] = -1;
@@ -2195,7 +2207,7 @@ __device__ void ph_upd_clamp(const Dev& D, const Range& r, Tally& t) {
     t.arc += 1;
     if (rs < 0) {
       const int a = -rs;
-      const int p = D.rev[arc];
+      const int p = D.urev ? D.urev[i] : D.rev[arc];
       D.res[arc] = 0;
       D.rres[p] = 0;
       D.rres[arc] -= a;

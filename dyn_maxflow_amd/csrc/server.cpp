@@ -289,7 +289,7 @@ int dmf_server_open(dmf_server* S, const dmf_csr* g, int32_t s, int32_t t, int32
                     int32_t* session, int64_t* static_flow) {
   return guarded([&]() -> int {
     if (!S || !g || !session || max_batch < 1) return DMF_EINVAL;
-    const int64_t need = dmf_engine_footprint(g->n, g->m) + static_cast<int64_t>(max_batch) * 16;
+    const int64_t need = dmf_engine_footprint(g->n, g->m) + static_cast<int64_t>(max_batch) * 24;  // update slots
     auto ss = std::make_unique<Session>();  // before the reservation, which nothing below throws past
     {
       std::lock_guard<std::mutex> lk(S->mu);

@@ -368,12 +368,13 @@ def test_lazy_start_keeps_results(gpu, mode, interval):
 
 def degree_boundary_graph(dmf, seed=5):
     """s -> hubs whose adjacency degree (CSR row, reverse arcs included) is
-    exactly 16/17, 64/65, 256/257 and 512/513: the default lane threshold, the
-    BFS wide threshold, one 256-arc chunk vs two, two chunks vs three.  Hubs
-    feed a pool of leaves that drain into t and into each other, so every hub
-    is active in the push phases and on the BFS frontier."""
+    exactly 4/5, 16/17, 64/65, 256/257 and 512/513: the default lane threshold
+    of push and repair, the topology-driven lane threshold (the static wave
+    list), the BFS wide threshold, one 256-arc chunk vs two, two chunks vs
+    three.  Hubs feed a pool of leaves that drain into t and into each other,
+    so every hub is active in the push phases and on the BFS frontier."""
     rng = np.random.default_rng(seed)
-    degs = [16, 17, 64, 65, 256, 257, 512, 513]
+    degs = [4, 5, 16, 17, 64, 65, 256, 257, 512, 513]
     s, t = 0, 1
     hubs = list(range(2, 2 + len(degs)))
     nleaf = 700
